@@ -1,0 +1,100 @@
+/*
+ * msw_plan.h -- flat C-ABI of the MI355X sweep library (libminisweep.so).
+ *
+ * Plain pointers and sizes; every array argument is a DEVICE pointer (HBM) and
+ * every stream is a hipStream_t passed as void* (NULL = the null stream).
+ * All functions return 0 on success and a nonzero MSW_ERR_* code otherwise;
+ * msw_last_error() gives the message.  These are the entry points the drop-in
+ * Sweeper_* API (sweeper.h) is built on, and what a binding from another
+ * language calls.
+ *
+ * Array layouts are the reference's (src/2_sweeper_base/array_accessors_kernels.h):
+ *   state      v[iz][ie][iy][ix][iu][im]                       (:82-88)
+ *   a_from_m   A[octant][im][ia], im stride = nm               (:346-349)
+ *   m_from_a   M[octant][ia][im], im stride = nm               (:410-413)
+ */
+#ifndef MSW_PLAN_H
+#define MSW_PLAN_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MSW_OK = 0,
+  MSW_ERR_INVALID = 1,   /* bad dimensions / arguments */
+  MSW_ERR_HIP = 2,       /* a HIP runtime call failed */
+  MSW_ERR_UNSUPPORTED = 3,
+  MSW_ERR_NOMEM = 4
+};
+
+/* Arithmetic of a sweep. */
+enum {
+  MSW_PREC_FP64 = 0,        /* plain FP64, parity path */
+  MSW_PREC_MIXED_FP32 = 1,  /* FP32 contractions + FP32 face recurrence */
+  MSW_PREC_MIXED_BF16 = 2   /* BF16 contractions (FP32 accumulate) + FP32 faces */
+};
+
+/* Problem geometry of one sweep (reference Dimensions + runtime nu). */
+typedef struct {
+  int ncell_x, ncell_y, ncell_z; /* grid cells */
+  int ne;                        /* energy groups */
+  int nm;                        /* moments */
+  int na;                        /* angles per octant */
+  int nu;                        /* unknowns per cell/moment/group */
+} MswDims;
+
+typedef struct MswPlan MswPlan;
+
+/* Last error message of the calling thread ("" if none). */
+const char* msw_last_error(void);
+
+/* Number of doubles of a state vector: ncx*ncy*ncz*ne*nm*nu. */
+size_t msw_size_state(const MswDims* d);
+
+/* Flop count of niterations sweeps, as the reference counts it
+ * (src/4_driver/runner.c:162-166). */
+double msw_flops(const MswDims* d, int niterations);
+
+/* Create a sweep plan on the current device: schedule, face buffers and
+ * the per-octant matrix staging.  Replaces the allocation half of
+ * Sweeper_create (src/3_sweeper/sweeper_kba_c.h:44-287). */
+int msw_plan_create(const MswDims* d, int precision, MswPlan** plan);
+
+/* Device bytes the plan holds. */
+size_t msw_plan_workspace_bytes(const MswPlan* plan);
+
+/* Number of kernel launches one sweep takes. */
+int msw_plan_num_launches(const MswPlan* plan);
+
+/* One sweep vo <- S(vi), all arrays resident in HBM, enqueued on stream.
+ * Replaces Sweeper_sweep (src/3_sweeper/sweeper_kba_c.h:588-845).
+ * vo need not be zeroed; vi is not modified. */
+int msw_plan_sweep(MswPlan* plan, const double* vi, double* vo,
+                   const double* a_from_m, const double* m_from_a,
+                   void* stream);
+
+/* Release the plan.  Replaces Sweeper_destroy (sweeper_kba_c.h:292-329). */
+int msw_plan_destroy(MswPlan* plan);
+
+/* normsq = sum vo^2, normsqdiff = sum (vi-vo)^2 in FP64 on the device,
+ * written to out2[0], out2[1] (device pointer).  Deterministic order.
+ * Replaces get_state_norms (src/2_sweeper_base/array_operations.c:69-110). */
+int msw_state_norms(const MswDims* d, const double* vi, const double* vo,
+                    double* out2, void* stream);
+
+/* Fill v with the manufactured input state (Quantities_init_state,
+ * src/3_sweeper/quantities_testing.h:60-90) on the device. */
+int msw_init_state(const MswDims* d, double* v, void* stream);
+
+/* Host: fill a_from_m, m_from_a (each 8*nm*na doubles, host memory) as
+ * Quantities_init_am_matrices_ (src/3_sweeper/quantities_testing_c.h:40-201). */
+int msw_init_matrices_host(int nm, int na, double* a_from_m, double* m_from_a);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

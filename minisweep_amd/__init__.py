@@ -1,0 +1,169 @@
+"""minisweep_amd -- MI355X-native sweep library, Python host binding.
+
+Thin ctypes binding over ``lib/libminisweep.so`` (the C-ABI declared in
+``include/msw_plan.h``).  Device buffers come from PyTorch (plumbing only:
+HBM allocation and streams); all sweep arithmetic runs in the library's HIP
+kernels.  There is no CPU fallback: if the library is missing every entry
+point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libminisweep.so")
+
+PREC_FP64 = 0
+PREC_MIXED_FP32 = 1
+PREC_MIXED_BF16 = 2
+
+NOCTANT = 8
+
+
+class MswDims(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in
+                ("ncell_x", "ncell_y", "ncell_z", "ne", "nm", "na", "nu")]
+
+    def tuple(self):
+        return (self.ncell_x, self.ncell_y, self.ncell_z, self.ne, self.nm,
+                self.na, self.nu)
+
+
+class SweepError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load the native library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SweepError(
+                f"native library missing: {LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        dims_p = ctypes.POINTER(MswDims)
+        L.msw_last_error.restype = ctypes.c_char_p
+        L.msw_size_state.argtypes = [dims_p]
+        L.msw_size_state.restype = ctypes.c_size_t
+        L.msw_flops.argtypes = [dims_p, ctypes.c_int]
+        L.msw_flops.restype = ctypes.c_double
+        L.msw_plan_create.argtypes = [dims_p, ctypes.c_int, ctypes.POINTER(vp)]
+        L.msw_plan_workspace_bytes.argtypes = [vp]
+        L.msw_plan_workspace_bytes.restype = ctypes.c_size_t
+        L.msw_plan_num_launches.argtypes = [vp]
+        L.msw_plan_sweep.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.msw_plan_destroy.argtypes = [vp]
+        L.msw_state_norms.argtypes = [dims_p, vp, vp, vp, vp]
+        L.msw_init_state.argtypes = [dims_p, vp, vp]
+        L.msw_init_matrices_host.argtypes = [ctypes.c_int, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_double),
+                                             ctypes.POINTER(ctypes.c_double)]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        msg = lib().msw_last_error().decode(errors="replace")
+        raise SweepError(f"{what} failed (code {rc}): {msg}")
+
+
+def dims(ncell_x, ncell_y, ncell_z, ne, nm, na, nu=4) -> MswDims:
+    return MswDims(ncell_x, ncell_y, ncell_z, ne, nm, na, nu)
+
+
+def size_state(d: MswDims) -> int:
+    return int(lib().msw_size_state(ctypes.byref(d)))
+
+
+def flops(d: MswDims, niterations: int = 1) -> float:
+    return float(lib().msw_flops(ctypes.byref(d), niterations))
+
+
+def init_matrices(nm: int, na: int):
+    """a_from_m, m_from_a as numpy float64 (Quantities_init_am_matrices_)."""
+    a = np.zeros(NOCTANT * nm * na)
+    m = np.zeros(NOCTANT * nm * na)
+    dp = ctypes.POINTER(ctypes.c_double)
+    _check(lib().msw_init_matrices_host(nm, na, a.ctypes.data_as(dp),
+                                        m.ctypes.data_as(dp)),
+           "msw_init_matrices_host")
+    return a, m
+
+
+def _stream_ptr(stream) -> Optional[int]:
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return int(stream.cuda_stream) if hasattr(stream, "cuda_stream") else int(stream)
+
+
+def _dptr(t) -> int:
+    import torch
+    assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
+    return int(t.data_ptr())
+
+
+class Plan:
+    """A sweep plan: schedule + face buffers for one problem shape."""
+
+    def __init__(self, d: MswDims, precision: int = PREC_FP64):
+        self.d = d
+        self.precision = precision
+        h = ctypes.c_void_p()
+        _check(lib().msw_plan_create(ctypes.byref(d), precision, ctypes.byref(h)),
+               "msw_plan_create")
+        self._h = h
+
+    @property
+    def workspace_bytes(self) -> int:
+        return int(lib().msw_plan_workspace_bytes(self._h))
+
+    @property
+    def num_launches(self) -> int:
+        return int(lib().msw_plan_num_launches(self._h))
+
+    def sweep(self, vi, vo, a_from_m, m_from_a, stream=None):
+        """vo <- S(vi); all tensors float64 on the current device."""
+        _check(lib().msw_plan_sweep(self._h, _dptr(vi), _dptr(vo), _dptr(a_from_m),
+                                    _dptr(m_from_a), _stream_ptr(stream)),
+               "msw_plan_sweep")
+
+    def close(self):
+        if self._h:
+            lib().msw_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def state_norms(d: MswDims, vi, vo, stream=None):
+    """(normsq, normsqdiff) in FP64 on the device."""
+    import torch
+    out = torch.empty(2, dtype=torch.float64, device=vi.device)
+    _check(lib().msw_state_norms(ctypes.byref(d), _dptr(vi), _dptr(vo), _dptr(out),
+                                 _stream_ptr(stream)), "msw_state_norms")
+    o = out.cpu()
+    return float(o[0]), float(o[1])
+
+
+def init_state(d: MswDims, v, stream=None):
+    _check(lib().msw_init_state(ctypes.byref(d), _dptr(v), _stream_ptr(stream)),
+           "msw_init_state")
+
+
+def state_shape(d: MswDims):
+    """Shape of the reference state layout v[iz][ie][iy][ix][iu][im]."""
+    return (d.ncell_z, d.ne, d.ncell_y, d.ncell_x, d.nu, d.nm)

@@ -1,0 +1,58 @@
+// msw_device.h -- device-side helpers shared by the HIP sweep kernels.
+//
+// The manufactured-problem formulas follow
+// src/3_sweeper/quantities_testing_kernels.h:52-214 of the reference; the
+// boundary-face value follows :219-332 and the solve :380-446.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace msw {
+
+constexpr int kNoct = 8;
+constexpr int kTile = 8;      // xy tile edge: one wave64 = 8x8 cells
+constexpr int kSkew = 2 * (kTile - 1);
+
+// One (octant, tile) visit in a launch.  Tile coordinates are in the
+// octant's own frame (upwind corner at 0,0).  first != 0: this visit is the
+// first of the sweep to reach the tile, so it writes vo instead of adding.
+struct TileItem {
+  int octant;
+  int first;
+  int tx;
+  int ty;
+};
+
+__host__ __device__ __forceinline__ int sf_space_exp(int x, int y, int z) {
+  return (x + 3 * y + 7 * z + 2) & 3;
+}
+__host__ __device__ __forceinline__ int sf_energy(int ie) {
+  return 1 << (((ie * 1366 + 150889) % 714025) & 3);
+}
+__host__ __device__ __forceinline__ int sf_unknown(int iu) {
+  return 1 << (((iu * 741 + 66037) % 312500) & 3);
+}
+
+// The cell solve.  Contraction is off: the reference rounds each product and
+// sum separately, and its exact manufactured answer depends on that.
+//   cz = xfluxweight * Sz^-1, cy = yfluxweight * Sy^-1, cx = zfluxweight(ia) * Sx^-1
+// are exact power-of-two rescalings of the reference's coefficients, so the
+// roundings are the reference's ones.
+__device__ __forceinline__ double solve_fp64(double vs, double fz, double fy,
+                                             double fx, double s_r, double cz,
+                                             double cy, double cx, double oct_r,
+                                             double s) {
+#pragma clang fp contract(off)
+  return (vs * s_r + ((fz * cz + fy * cy) + fx * cx) * oct_r) * s;
+}
+
+__device__ __forceinline__ float solve_fp32(float vs, float fz, float fy,
+                                            float fx, float s_r, float cz,
+                                            float cy, float cx, float oct_r,
+                                            float s) {
+#pragma clang fp contract(off)
+  return (vs * s_r + ((fz * cz + fy * cy) + fx * cx) * oct_r) * s;
+}
+
+}  // namespace msw

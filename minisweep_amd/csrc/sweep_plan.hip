@@ -1,0 +1,690 @@
+// sweep_plan.hip -- the FP64 sweep on gfx950 and the flat plan C-ABI.
+//
+// Work decomposition (the hyperplane wavefront, MI355X-first):
+//   * The sweep of one octant is a 3-D wavefront.  One wave64 owns an 8x8
+//     xy tile of cells for one (octant, energy group, unknown) and walks the
+//     whole z extent; lane (xl, yl) handles cell z = t - xl - yl at step t,
+//     so the 64 lanes of a step lie on the diagonal hyperplane
+//     xl + yl + z = t of the tile.
+//   * Each lane keeps the outgoing face of its last cell for all its angles in
+//     registers (the reference's three face arrays hold the same value,
+//     quantities_testing_kernels.h:437-443).  The z face stays in the lane,
+//     the x and y faces move one lane per step (__shfl_up by 1 and by 8).
+//     Faces crossing a tile edge go through HBM face buffers and are staged
+//     in LDS once per step.
+//   * m2a and a2m are per-lane dense contractions with the octant's
+//     matrices in scalar registers (wave-uniform), FMA in FP64.
+//   * Tiles are visited in a host-built schedule: per launch, a set of
+//     (octant, tile) items whose upwind tiles finished in earlier launches,
+//     with no two items on the same tile, so every launch updates vo without
+//     atomics and the result is deterministic.  All 8 octants run together.
+//   * Energy groups and unknowns are fully independent; they fill the grid.
+//
+// Reference: the per-cell work is Sweeper_sweep_cell
+// (src/3_sweeper/sweeper_kba_c_kernels.h:30-555); the hyperplane form is
+// Sweeper_in_gridcell (src/3_sweeper/sweeper_acc_c.h:262-436); the schedule
+// plays the role of StepScheduler_stepinfo (src/3_sweeper/stepscheduler_kba.c:121-284).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "msw_device.h"
+#include "../../include/msw_plan.h"
+
+namespace msw {
+
+static thread_local std::string g_err;
+
+static int set_err(int code, const char* fmt, const char* a = "", long b = 0) {
+  char buf[512];
+  snprintf(buf, sizeof(buf), fmt, a, b);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(call)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return set_err(MSW_ERR_HIP, "%s (line %ld)", hipGetErrorString(e_),    \
+                     (long)__LINE__);                                        \
+  } while (0)
+
+// Read-only data that is uniform across a wave is loaded through the
+// constant address space, which the backend turns into scalar loads.
+typedef const __attribute__((address_space(4))) double* cdptr;
+
+struct SweepParams {
+  const double* vi;
+  double* vo;
+  const double* at;  // a_from_m transposed: [8][na][nm]
+  const double* mf;  // m_from_a:            [8][na][nm]
+  double* fyz;       // x-direction faces  [8][ne][nu][ny][nz][na] (octant frame)
+  double* fxz;       // y-direction faces  [8][ne][nu][nx][nz][na] (octant frame)
+  const TileItem* items;
+  int nx, ny, nz, ne, na, nu;
+  int ntx, nty;
+};
+
+// ---------------------------------------------------------------------------
+// FP64 tile kernel.  NM: moments (compile time), NAC: angles per wave.
+// blockDim.x = 64 * nwa, nwa = ceil(na / NAC) waves split the angles.
+template <int NM, int NAC>
+__global__ __launch_bounds__(512) void k_sweep_fp64(SweepParams p,
+                                                    int item_base) {
+  constexpr int LROW = NAC + 1;  // padded LDS row (bank spread)
+  const int lane = threadIdx.x & 63;
+  // wave index is uniform within a wave: keep it (and the angle chunk) scalar
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nwa = blockDim.x >> 6;
+  int b = blockIdx.x;
+  const int iu = b % p.nu;
+  b /= p.nu;
+  const int ie = b % p.ne;
+  b /= p.ne;
+  const TileItem it = p.items[item_base + b];
+  const int oct = it.octant;
+  const int dx = (oct & 1) ? -1 : 1;
+  const int dy = (oct & 2) ? -1 : 1;
+  const int dz = (oct & 4) ? -1 : 1;
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+
+  const int xl = lane & (kTile - 1), yl = lane >> 3;
+  const int xp = it.tx * kTile + xl, yp = it.ty * kTile + yl;
+  const bool in_xy = xp < nx && yp < ny;
+  const int x = dx > 0 ? xp : nx - 1 - xp;
+  const int y = dy > 0 ? yp : ny - 1 - yp;
+
+  const int ia0 = wave * NAC;
+  const int nac = min(NAC, p.na - ia0);
+
+  extern __shared__ double lds[];
+  double* lx = lds + (size_t)wave * (2 * kTile * LROW);  // x inflow [yl][ia]
+  double* ly = lx + kTile * LROW;                       // y inflow [xl][ia]
+  double* red = lds + (size_t)nwa * (2 * kTile * LROW); // [nwa][NM][64]
+
+  const double* __restrict__ at = p.at + ((size_t)oct * p.na + ia0) * NM;
+  const double* __restrict__ mf = p.mf + ((size_t)oct * p.na + ia0) * NM;
+
+  const double oct_s = (double)(1 + oct);
+  const double oct_r = 1.0 / oct_s;
+  const double eu_oct = (double)sf_energy(ie) * (double)sf_unknown(iu) * oct_s;
+
+  const size_t face_ouy = (((size_t)oct * p.ne + ie) * p.nu + iu);
+  double* __restrict__ fyz = p.fyz + face_ouy * ny * nz * p.na;  // [yp][zp][ia]
+  double* __restrict__ fxz = p.fxz + face_ouy * nx * nz * p.na;  // [xp][zp][ia]
+  const int xb = dx > 0 ? -1 : nx;  // out-of-domain coordinate of each inflow face
+  const int yb = dy > 0 ? -1 : ny;
+  const int zb = dz > 0 ? -1 : nz;
+
+  double F[NAC];
+#pragma unroll
+  for (int i = 0; i < NAC; ++i) F[i] = 0.0;
+
+  const size_t sstride_z = (size_t)p.ne * ny * nx * p.nu * NM;
+  const size_t sstride_e = (size_t)ny * nx * p.nu * NM;
+  const size_t scell = ((size_t)y * nx + x) * p.nu * NM + (size_t)iu * NM;
+
+  const int nsteps = nz + kSkew;
+  for (int t = 0; t < nsteps; ++t) {
+    const int zp = t - xl - yl;
+    const bool active = in_xy && zp >= 0 && zp < nz;
+    const int z = dz > 0 ? zp : nz - 1 - zp;
+
+    // ---- stage this step's x and y inflow faces in LDS --------------------
+    // Row r of lx: the x-inflow of lane (0, r), cell (yp = ty*8+r, zp = t-r).
+    // Row c of ly: the y-inflow of lane (c, 0), cell (xp = tx*8+c, zp = t-c).
+    {
+      const int r = lane >> 3, s = lane & 7;
+      {
+        const int zq = t - r, yq = it.ty * kTile + r;
+        if (zq >= 0 && zq < nz && yq < ny) {
+          if (it.tx == 0) {
+            const int yg = dy > 0 ? yq : ny - 1 - yq;
+            const int zg = dz > 0 ? zq : nz - 1 - zq;
+            const double sc = ldexp(eu_oct, sf_space_exp(xb, yg, zg));
+            for (int i = s; i < nac; i += 8)
+              lx[r * LROW + i] = (double)((1 + ia0 + i) << (i & 7)) * sc;
+          } else {
+            const double* src = fyz + ((size_t)yq * nz + zq) * p.na + ia0;
+            for (int i = s; i < nac; i += 8) lx[r * LROW + i] = src[i];
+          }
+        }
+      }
+      {
+        const int zq = t - r, xq = it.tx * kTile + r;
+        if (zq >= 0 && zq < nz && xq < nx) {
+          if (it.ty == 0) {
+            const int xg = dx > 0 ? xq : nx - 1 - xq;
+            const int zg = dz > 0 ? zq : nz - 1 - zq;
+            const double sc = ldexp(eu_oct, sf_space_exp(xg, yb, zg));
+            for (int i = s; i < nac; i += 8)
+              ly[r * LROW + i] = (double)((1 + ia0 + i) << (i & 7)) * sc;
+          } else {
+            const double* src = fxz + ((size_t)xq * nz + zq) * p.na + ia0;
+            for (int i = s; i < nac; i += 8) ly[r * LROW + i] = src[i];
+          }
+        }
+      }
+    }
+    // z inflow: a lane entering the grid (zp == 0) starts from the boundary.
+    if (in_xy && zp == 0) {
+      const double sc = ldexp(eu_oct, sf_space_exp(x, y, zb));
+#pragma unroll
+      for (int i = 0; i < NAC; ++i)
+        F[i] = (double)((1 + ia0 + i) << (i & 7)) * sc;
+    }
+    __builtin_amdgcn_wave_barrier();
+
+    // ---- cell constants ----------------------------------------------------
+    double v[NM];
+    const size_t cbase = (size_t)(active ? z : 0) * sstride_z +
+                         (size_t)ie * sstride_e + scell;
+    if (active) {
+      const double2* src = reinterpret_cast<const double2*>(p.vi + cbase);
+#pragma unroll
+      for (int i = 0; i < NM / 2; ++i) {
+        const double2 w = src[i];
+        v[2 * i] = w.x;
+        v[2 * i + 1] = w.y;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < NM; ++i) v[i] = 0.0;
+    }
+    const int e_s = sf_space_exp(x, y, z);
+    const double s = ldexp(1.0, e_s);
+    const double s_r = ldexp(1.0, -e_s);
+    const double sx_r = ldexp(1.0, -sf_space_exp(x - dx, y, z));
+    const double cy = ldexp(0.25, -sf_space_exp(x, y - dy, z));
+    const double cz = ldexp(0.5, -sf_space_exp(x, y, z - dz));
+
+    double acc[NM];
+#pragma unroll
+    for (int i = 0; i < NM; ++i) acc[i] = 0.0;
+
+    // The octant's matrices are read with scalar loads (constant address
+    // space: wave-uniform, through the scalar cache) every step; an opaque
+    // copy of the pointers keeps the compiler from hoisting 2*NAC*NM doubles
+    // out of the step loop into registers.
+    cdptr at_s = (cdptr)(at);
+    cdptr mf_s = (cdptr)(mf);
+    __asm__ volatile("" : "+s"(at_s), "+s"(mf_s));
+
+#pragma unroll
+    for (int ia = 0; ia < NAC; ++ia) {
+      if (ia < nac) {
+        double vs = 0.0;
+#pragma unroll
+        for (int im = 0; im < NM; ++im) vs = fma(at_s[ia * NM + im], v[im], vs);
+        const double fxn = __shfl_up(F[ia], 1);
+        const double fyn = __shfl_up(F[ia], 8);
+        const double fx = xl == 0 ? lx[yl * LROW + ia] : fxn;
+        const double fy = yl == 0 ? ly[xl * LROW + ia] : fyn;
+        // zfluxweight(ia) = 1/4 - 2^-(ia&7), exact; ia0 is a multiple of 8.
+        const double cx = (0.25 - ldexp(1.0, -(ia & 7))) * sx_r;
+        const double r = solve_fp64(vs, F[ia], fy, fx, s_r, cz, cy, cx, oct_r, s);
+        F[ia] = r * oct_s;
+#pragma unroll
+        for (int im = 0; im < NM; ++im) acc[im] = fma(mf_s[ia * NM + im], r, acc[im]);
+      }
+      // keep each angle's scalar matrix loads next to their use
+      __builtin_amdgcn_sched_barrier(0);
+    }
+
+    // ---- outflow faces for the downstream tiles ----------------------------
+    if (active && xl == kTile - 1 && it.tx + 1 < p.ntx) {
+      double* dst = fyz + ((size_t)yp * nz + zp) * p.na + ia0;
+#pragma unroll
+      for (int i = 0; i < NAC; ++i)
+        if (i < nac) dst[i] = F[i];
+    }
+    if (active && yl == kTile - 1 && it.ty + 1 < p.nty) {
+      double* dst = fxz + ((size_t)xp * nz + zp) * p.na + ia0;
+#pragma unroll
+      for (int i = 0; i < NAC; ++i)
+        if (i < nac) dst[i] = F[i];
+    }
+
+    // ---- a2m result -> vo ----------------------------------------------------
+    if (nwa == 1) {
+      if (active) {
+        double2* dst = reinterpret_cast<double2*>(p.vo + cbase);
+        if (it.first) {
+#pragma unroll
+          for (int i = 0; i < NM / 2; ++i) dst[i] = make_double2(acc[2 * i], acc[2 * i + 1]);
+        } else {
+#pragma unroll
+          for (int i = 0; i < NM / 2; ++i) {
+            double2 o = dst[i];
+            o.x += acc[2 * i];
+            o.y += acc[2 * i + 1];
+            dst[i] = o;
+          }
+        }
+      }
+    } else {
+      // partial sums of the angle chunks, summed in chunk order
+#pragma unroll
+      for (int im = 0; im < NM; ++im) red[((size_t)wave * NM + im) * 64 + lane] = acc[im];
+      __syncthreads();
+      if (active) {
+        for (int im = wave; im < NM; im += nwa) {
+          double sum = red[(size_t)im * 64 + lane];
+          for (int w = 1; w < nwa; ++w) sum += red[((size_t)w * NM + im) * 64 + lane];
+          double* dst = p.vo + cbase + im;
+          *dst = it.first ? sum : *dst + sum;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// a_from_m [8][im][ia] -> at [8][ia][im]
+__global__ void k_transpose_afm(const double* __restrict__ afm,
+                                double* __restrict__ at, int nm, int na) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kNoct * nm * na) return;
+  const int ia = i % na;
+  const int im = (i / na) % nm;
+  const int oct = i / (na * nm);
+  at[((size_t)oct * na + ia) * nm + im] = afm[i];
+}
+
+// Manufactured state v = (1+im) S(x,y,z) E(ie) U(iu), layout [iz][ie][iy][ix][iu][im].
+__global__ void k_init_state(double* __restrict__ v, int nx, int ny, int nz,
+                             int ne, int nm, int nu, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    size_t r = i;
+    const int im = r % nm; r /= nm;
+    const int iu = r % nu; r /= nu;
+    const int ix = r % nx; r /= nx;
+    const int iy = r % ny; r /= ny;
+    const int ie = r % ne; r /= ne;
+    const int iz = (int)r;
+    v[i] = (double)(1 + im) * ldexp(1.0, sf_space_exp(ix, iy, iz)) *
+           (double)sf_energy(ie) * (double)sf_unknown(iu);
+  }
+}
+
+// Deterministic two-pass FP64 norms: fixed block partials, then one block.
+constexpr int kNormBlocks = 1024;
+constexpr int kNormThreads = 256;
+
+__global__ __launch_bounds__(kNormThreads) void k_norms_partial(
+    const double* __restrict__ vi, const double* __restrict__ vo, size_t n,
+    double* __restrict__ part) {
+  __shared__ double s0[kNormThreads], s1[kNormThreads];
+  double a = 0.0, b = 0.0;
+  for (size_t i = blockIdx.x * (size_t)kNormThreads + threadIdx.x; i < n;
+       i += (size_t)kNormBlocks * kNormThreads) {
+    const double o = vo[i];
+    const double d = vi[i] - o;
+    a = fma(o, o, a);
+    b = fma(d, d, b);
+  }
+  s0[threadIdx.x] = a;
+  s1[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = kNormThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      s0[threadIdx.x] += s0[threadIdx.x + w];
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s0[0];
+    part[kNormBlocks + blockIdx.x] = s1[0];
+  }
+}
+
+__global__ __launch_bounds__(kNormThreads) void k_norms_final(
+    const double* __restrict__ part, double* __restrict__ out2) {
+  __shared__ double s0[kNormThreads], s1[kNormThreads];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < kNormBlocks; i += kNormThreads) {
+    a += part[i];
+    b += part[kNormBlocks + i];
+  }
+  s0[threadIdx.x] = a;
+  s1[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = kNormThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      s0[threadIdx.x] += s0[threadIdx.x + w];
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out2[0] = s0[0];
+    out2[1] = s1[0];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side schedule: list scheduling of (octant, tile) visits.
+//
+// Octant o's tile (i, j) in its own frame needs (i-1, j) and (i, j-1) done.
+// A launch takes every ready visit whose physical tile is not yet taken in
+// that launch, octants with the most remaining visits first.  Returns the
+// launches in order; marks the first visit of every physical tile.
+std::vector<std::vector<TileItem>> build_schedule(int ntx, int nty) {
+  std::vector<std::vector<TileItem>> launches;
+  std::vector<std::vector<int>> done(kNoct, std::vector<int>(ntx * nty, 0));
+  std::vector<int> remaining(kNoct, ntx * nty);
+  std::vector<int> touched(ntx * nty, 0);
+  int left = kNoct * ntx * nty;
+  while (left > 0) {
+    std::vector<int> order(kNoct);
+    for (int o = 0; o < kNoct; ++o) order[o] = o;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return remaining[a] > remaining[b]; });
+    std::vector<int> used(ntx * nty, 0);
+    std::vector<TileItem> items;
+    for (int o : order) {
+      // ready visits, nearest the upwind corner first
+      for (int d = 0; d <= ntx + nty - 2; ++d)
+        for (int i = 0; i < ntx; ++i) {
+          const int j = d - i;
+          if (j < 0 || j >= nty) continue;
+          if (done[o][i + ntx * j]) continue;
+          if (i > 0 && done[o][(i - 1) + ntx * j] != 1) continue;
+          if (j > 0 && done[o][i + ntx * (j - 1)] != 1) continue;
+          const int gx = (o & 1) ? ntx - 1 - i : i;
+          const int gy = (o & 2) ? nty - 1 - j : j;
+          const int g = gx + ntx * gy;
+          if (used[g]) continue;
+          used[g] = 1;
+          TileItem t;
+          t.octant = o;
+          t.tx = i;
+          t.ty = j;
+          t.first = touched[g] ? 0 : 1;
+          items.push_back(t);
+        }
+    }
+    for (const TileItem& t : items) {
+      done[t.octant][t.tx + ntx * t.ty] = 2;  // finished in this launch
+      const int gx = (t.octant & 1) ? ntx - 1 - t.tx : t.tx;
+      const int gy = (t.octant & 2) ? nty - 1 - t.ty : t.ty;
+      touched[gx + ntx * gy] = 1;
+      remaining[t.octant]--;
+      left--;
+    }
+    for (int o = 0; o < kNoct; ++o)
+      for (auto& v : done[o])
+        if (v == 2) v = 1;
+    launches.push_back(items);
+  }
+  return launches;
+}
+
+}  // namespace msw
+
+using namespace msw;
+
+struct MswPlan {
+  MswDims d;
+  int precision;
+  int nac, nwa, ntx, nty;
+  size_t lds_bytes;
+  double* at;  // a_from_m transposed [8][na][nm] (+ padding)
+  double* mf;  // m_from_a copy       [8][na][nm] (+ padding)
+  double* fyz;
+  double* fxz;
+  double* norm_part;
+  TileItem* items;
+  std::vector<int> launch_off, launch_cnt;
+  size_t bytes;
+};
+
+template <int NM, int NAC>
+static int launch_all(MswPlan* p, const SweepParams& sp, hipStream_t st) {
+  const dim3 block(64 * p->nwa);
+  for (size_t l = 0; l < p->launch_off.size(); ++l) {
+    const dim3 grid((unsigned)p->launch_cnt[l] * p->d.ne * p->d.nu);
+    hipLaunchKernelGGL((k_sweep_fp64<NM, NAC>), grid, block, p->lds_bytes, st, sp,
+                       p->launch_off[l]);
+  }
+  HIP_TRY(hipGetLastError());
+  return MSW_OK;
+}
+
+template <int NM>
+static int launch_nm(MswPlan* p, const SweepParams& sp, hipStream_t st) {
+  switch (p->nac) {
+    case 8: return launch_all<NM, 8>(p, sp, st);
+    case 16: return launch_all<NM, 16>(p, sp, st);
+    default: return launch_all<NM, 32>(p, sp, st);
+  }
+}
+
+extern "C" {
+
+const char* msw_last_error(void) { return g_err.c_str(); }
+
+size_t msw_size_state(const MswDims* d) {
+  return (size_t)d->ncell_x * d->ncell_y * d->ncell_z * d->ne * d->nm * d->nu;
+}
+
+double msw_flops(const MswDims* d, int niterations) {
+  const double ss = (double)msw_size_state(d);
+  const double ssa = (double)d->ncell_x * d->ncell_y * d->ncell_z * d->ne *
+                     d->na * d->nu * kNoct;
+  return niterations * (ss * kNoct * 2. * d->na + ssa * 12. + ss * kNoct * 2. * d->na);
+}
+
+static int check_dims(const MswDims* d) {
+  if (!d) return set_err(MSW_ERR_INVALID, "null dims%s%ld");
+  if (d->ncell_x <= 0 || d->ncell_y <= 0 || d->ncell_z <= 0 || d->ne <= 0 ||
+      d->nm <= 0 || d->na <= 0 || d->nu <= 0)
+    return set_err(MSW_ERR_INVALID, "all dimensions must be positive%s%ld");
+  return MSW_OK;
+}
+
+// Angles per wave.  A lane keeps one outgoing face value per angle in
+// registers; 32 angles (64 VGPRs) leave room for the moments and the a2m
+// accumulators without spilling, so larger na is split over several waves.
+static int pick_nac(int na) {
+  if (na <= 8) return 8;
+  if (na <= 16) return 16;
+  return 32;
+}
+
+int msw_plan_create(const MswDims* d, int precision, MswPlan** out) {
+  if (!out) return set_err(MSW_ERR_INVALID, "null plan pointer%s%ld");
+  *out = nullptr;
+  int rc = check_dims(d);
+  if (rc) return rc;
+  if (precision != MSW_PREC_FP64)
+    return set_err(MSW_ERR_UNSUPPORTED, "precision %s%ld not built yet", "", precision);
+  if (d->nm != 4 && d->nm != 8 && d->nm != 16)
+    return set_err(MSW_ERR_UNSUPPORTED, "nm must be 4, 8 or 16 (got %s%ld)", "", d->nm);
+  MswPlan* p = new MswPlan();
+  p->d = *d;
+  p->precision = precision;
+  p->nac = pick_nac(d->na);
+  p->nwa = (d->na + p->nac - 1) / p->nac;
+  if (p->nwa > 8) {
+    delete p;
+    return set_err(MSW_ERR_UNSUPPORTED, "na above 256 unsupported (na=%s%ld)", "", d->na);
+  }
+  p->ntx = (d->ncell_x + kTile - 1) / kTile;
+  p->nty = (d->ncell_y + kTile - 1) / kTile;
+  p->lds_bytes = sizeof(double) * ((size_t)p->nwa * 2 * kTile * (p->nac + 1) +
+                                   (p->nwa > 1 ? (size_t)p->nwa * d->nm * 64 : 0));
+  auto launches = build_schedule(p->ntx, p->nty);
+  std::vector<TileItem> flat;
+  for (auto& l : launches) {
+    p->launch_off.push_back((int)flat.size());
+    p->launch_cnt.push_back((int)l.size());
+    flat.insert(flat.end(), l.begin(), l.end());
+  }
+  // matrix copies padded by one chunk: no scalar load can leave the buffer
+  const size_t nat = ((size_t)kNoct * d->na + 64) * d->nm;
+  const size_t nfyz = (size_t)kNoct * d->ne * d->nu * d->ncell_y * d->ncell_z * d->na;
+  const size_t nfxz = (size_t)kNoct * d->ne * d->nu * d->ncell_x * d->ncell_z * d->na;
+  p->bytes = sizeof(double) * (2 * nat + nfyz + nfxz + 2 * kNormBlocks) +
+             sizeof(TileItem) * flat.size();
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = hipMalloc(&p->at, sizeof(double) * nat);
+  if (e == hipSuccess) e = hipMalloc(&p->mf, sizeof(double) * nat);
+  if (e == hipSuccess) e = hipMemset(p->at, 0, sizeof(double) * nat);
+  if (e == hipSuccess) e = hipMemset(p->mf, 0, sizeof(double) * nat);
+  if (e == hipSuccess) e = hipMalloc(&p->fyz, sizeof(double) * nfyz);
+  if (e == hipSuccess) e = hipMalloc(&p->fxz, sizeof(double) * nfxz);
+  if (e == hipSuccess) e = hipMalloc(&p->norm_part, sizeof(double) * 2 * kNormBlocks);
+  if (e == hipSuccess) e = hipMalloc(&p->items, sizeof(TileItem) * flat.size());
+  if (e == hipSuccess)
+    e = hipMemcpy(p->items, flat.data(), sizeof(TileItem) * flat.size(),
+                  hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    msw_plan_destroy(p);
+    return set_err(e == hipErrorOutOfMemory ? MSW_ERR_NOMEM : MSW_ERR_HIP,
+                   "plan allocation failed: %s%ld", hipGetErrorString(e));
+  }
+  *out = p;
+  return MSW_OK;
+}
+
+size_t msw_plan_workspace_bytes(const MswPlan* p) { return p ? p->bytes : 0; }
+
+int msw_plan_num_launches(const MswPlan* p) {
+  return p ? (int)p->launch_off.size() : 0;
+}
+
+int msw_plan_destroy(MswPlan* p) {
+  if (!p) return MSW_OK;
+  if (p->at) (void)hipFree(p->at);
+  if (p->mf) (void)hipFree(p->mf);
+  if (p->fyz) (void)hipFree(p->fyz);
+  if (p->fxz) (void)hipFree(p->fxz);
+  if (p->norm_part) (void)hipFree(p->norm_part);
+  if (p->items) (void)hipFree(p->items);
+  delete p;
+  return MSW_OK;
+}
+
+int msw_plan_sweep(MswPlan* p, const double* vi, double* vo,
+                   const double* a_from_m, const double* m_from_a, void* stream) {
+  if (!p || !vi || !vo || !a_from_m || !m_from_a)
+    return set_err(MSW_ERR_INVALID, "null argument to msw_plan_sweep%s%ld");
+  if (vi == vo) return set_err(MSW_ERR_INVALID, "vi and vo must differ%s%ld");
+  hipStream_t st = (hipStream_t)stream;
+  const MswDims& d = p->d;
+  const int n = kNoct * d.nm * d.na;
+  hipLaunchKernelGGL(k_transpose_afm, dim3((n + 255) / 256), dim3(256), 0, st,
+                     a_from_m, p->at, d.nm, d.na);
+  HIP_TRY(hipMemcpyAsync(p->mf, m_from_a, sizeof(double) * n,
+                         hipMemcpyDeviceToDevice, st));
+  SweepParams sp;
+  sp.vi = vi;
+  sp.vo = vo;
+  sp.at = p->at;
+  sp.mf = p->mf;
+  sp.fyz = p->fyz;
+  sp.fxz = p->fxz;
+  sp.items = p->items;
+  sp.nx = d.ncell_x;
+  sp.ny = d.ncell_y;
+  sp.nz = d.ncell_z;
+  sp.ne = d.ne;
+  sp.na = d.na;
+  sp.nu = d.nu;
+  sp.ntx = p->ntx;
+  sp.nty = p->nty;
+  switch (d.nm) {
+    case 4: return launch_nm<4>(p, sp, st);
+    case 8: return launch_nm<8>(p, sp, st);
+    default: return launch_nm<16>(p, sp, st);
+  }
+}
+
+int msw_state_norms(const MswDims* d, const double* vi, const double* vo,
+                    double* out2, void* stream) {
+  int rc = check_dims(d);
+  if (rc) return rc;
+  if (!vi || !vo || !out2) return set_err(MSW_ERR_INVALID, "null norms argument%s%ld");
+  hipStream_t st = (hipStream_t)stream;
+  static thread_local double* part = nullptr;
+  if (!part) HIP_TRY(hipMalloc(&part, sizeof(double) * 2 * kNormBlocks));
+  hipLaunchKernelGGL(k_norms_partial, dim3(kNormBlocks), dim3(kNormThreads), 0, st,
+                     vi, vo, msw_size_state(d), part);
+  hipLaunchKernelGGL(k_norms_final, dim3(1), dim3(kNormThreads), 0, st, part, out2);
+  HIP_TRY(hipGetLastError());
+  return MSW_OK;
+}
+
+int msw_init_state(const MswDims* d, double* v, void* stream) {
+  int rc = check_dims(d);
+  if (rc) return rc;
+  if (!v) return set_err(MSW_ERR_INVALID, "null state%s%ld");
+  const size_t n = msw_size_state(d);
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_init_state, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+                     v, d->ncell_x, d->ncell_y, d->ncell_z, d->ne, d->nm, d->nu, n);
+  HIP_TRY(hipGetLastError());
+  return MSW_OK;
+}
+
+int msw_init_matrices_host(int nm, int na, double* A, double* M) {
+  if (nm <= 0 || na <= 0 || !A || !M)
+    return set_err(MSW_ERR_INVALID, "bad matrix init arguments%s%ld");
+  // Quantities_init_am_matrices_ (src/3_sweeper/quantities_testing_c.h:40-201):
+  // integer rows that send an affine vector in moments to an affine vector in
+  // angles and back, plus rows that annihilate affine vectors.
+  const size_t n = (size_t)kNoct * nm * na;
+  std::fill(A, A + n, 0.0);
+  std::fill(M, M + n, 0.0);
+  auto afm = [&](int o, int m, int a) -> double& { return A[a + (size_t)na * (m + (size_t)nm * o)]; };
+  auto mfa = [&](int o, int m, int a) -> double& { return M[m + (size_t)nm * (a + (size_t)na * o)]; };
+  for (int o = 0; o < kNoct; ++o) {
+    for (int i = 0; i < na; ++i) {
+      const int q = (i + 1) / nm, r = (i + 1) % nm;
+      afm(o, nm - 1, i) += q;
+      if (r) {
+        afm(o, 0, i) -= 1.0;
+        afm(o, r, i) += 1.0;
+      }
+    }
+    for (int m = 0; m < nm - 2; ++m)
+      for (int a = 0; a < na; ++a) {
+        const int rv = 21 + (m + nm * a) % 17;
+        afm(o, m, a) += -1.0 * rv;
+        afm(o, m + 1, a) += 2.0 * rv;
+        afm(o, m + 2, a) += -1.0 * rv;
+      }
+    for (int i = 0; i < nm; ++i) {
+      const int q = (i + 1) / na, r = (i + 1) % na;
+      mfa(o, i, na - 1) += q;
+      if (r) {
+        mfa(o, i, 0) -= 1.0;
+        mfa(o, i, r) += 1.0;
+      }
+    }
+    for (int m = 0; m < nm; ++m)
+      for (int a = 0; a < na - 2; ++a) {
+        const int rv = 37 + (m + nm * a) % 19;
+        mfa(o, m, a) += -1.0 * rv;
+        mfa(o, m, a + 1) += 2.0 * rv;
+        mfa(o, m, a + 2) += -1.0 * rv;
+      }
+    for (int m = 0; m < nm; ++m)
+      for (int a = 0; a < na; ++a) {
+        mfa(o, m, a) /= kNoct;
+        mfa(o, m, a) /= (double)(1 << (a & 7));
+      }
+  }
+  return MSW_OK;
+}
+
+}  // extern "C"

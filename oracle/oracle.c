@@ -1,0 +1,300 @@
+/*
+ * oracle.c -- CPU restatement of the minisweep sweep (TEST INFRASTRUCTURE).
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
+ * this file; the product library never links it.  See oracle.h for the list
+ * of reference functions restated here.
+ *
+ * Compiled with -ffp-contract=off: the reference is plain ANSI C built for
+ * x86-64 without FMA, and its manufactured answer (vo == vi exactly) depends
+ * on the separate roundings of the solve (quantities_testing_kernels.h:421-434).
+ */
+#include "oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+enum { NOCT = 8 };
+
+/* Octant bit b set => sweep runs downward along axis b
+ * (src/2_sweeper_base/definitions_kernels.h:46-60). */
+static int dir_of(int octant, int axis) { return (octant >> axis) & 1 ? -1 : +1; }
+
+/* quantities_testing_kernels.h:52-69 */
+static int sf_energy(int ie) {
+  int r = (ie * 1366 + 150889) % 714025;
+  return 1 << (r & 3);
+}
+
+/* quantities_testing_kernels.h:75-91 */
+static int sf_unknown(int iu) {
+  int r = (iu * 741 + 66037) % 312500;
+  return 1 << (r & 3);
+}
+
+/* quantities_testing_kernels.h:97-126: only the last assignment survives, so
+ * the factor is 2^((ix+3iy+7iz+2) & 3), evaluated in two's-complement int. */
+static int sf_space(int ix, int iy, int iz) {
+  int r = ix + 3 * iy + 7 * iz + 2;
+  return 1 << (r & 3);
+}
+
+/* quantities_testing_kernels.h:132-141 */
+static int sf_angle(int ia) { return 1 << (ia & 7); }
+
+/* quantities_testing_kernels.h:203-214 */
+static int sf_octant(int octant) { return 1 + octant; }
+
+size_t oracle_size_state(OracleDims d) {
+  return (size_t)d.ncell_x * d.ncell_y * d.ncell_z * d.ne * d.nm * d.nu;
+}
+
+double oracle_flops(OracleDims d, int niterations) {
+  const double ss = (double)oracle_size_state(d);
+  const double ssa = (double)d.ncell_x * d.ncell_y * d.ncell_z * d.ne * d.na *
+                     d.nu * NOCT;
+  /* quantities_testing_c.h:356-359: 3 + 3*NDIM flops per solve */
+  return niterations * (ss * NOCT * 2. * d.na + ssa * 12. + ss * NOCT * 2. * d.na);
+}
+
+/* state index (array_accessors_kernels.h:82-88), iz slowest */
+static size_t ind_state(const OracleDims* d, int ix, int iy, int iz, int ie,
+                        int im, int iu) {
+  return (size_t)im + d->nm * ((size_t)iu + d->nu * ((size_t)ix + d->ncell_x *
+         ((size_t)iy + d->ncell_y * ((size_t)ie + (size_t)d->ne * iz))));
+}
+
+/* Quantities_init_am_matrices_ (quantities_testing_c.h:40-201).  Same
+ * integer construction; our storage strides by the runtime nm. */
+void oracle_init_matrices(OracleDims d, double* A, double* M) {
+  const int nm = d.nm, na = d.na;
+  int oct, im, ia, i;
+  memset(A, 0, sizeof(double) * NOCT * nm * na);
+  memset(M, 0, sizeof(double) * NOCT * nm * na);
+#define AFM(o, m, a) A[(a) + na * ((m) + nm * (o))]
+#define MFA(o, m, a) M[(m) + nm * ((a) + na * (o))]
+  for (oct = 0; oct < NOCT; ++oct)
+    for (i = 0; i < na; ++i) {
+      const int quot = (i + 1) / nm, rem = (i + 1) % nm;
+      AFM(oct, nm - 1, i) += quot;
+      if (rem != 0) {
+        AFM(oct, 0, i) += -1.0;
+        AFM(oct, rem, i) += 1.0;
+      }
+    }
+  for (oct = 0; oct < NOCT; ++oct)
+    for (im = 0; im < nm - 2; ++im)
+      for (ia = 0; ia < na; ++ia) {
+        const int rv = 21 + (im + nm * ia) % 17;
+        AFM(oct, im + 0, ia) += -1.0 * rv;
+        AFM(oct, im + 1, ia) += 2.0 * rv;
+        AFM(oct, im + 2, ia) += -1.0 * rv;
+      }
+  for (oct = 0; oct < NOCT; ++oct)
+    for (i = 0; i < nm; ++i) {
+      const int quot = (i + 1) / na, rem = (i + 1) % na;
+      MFA(oct, i, na - 1) += quot;
+      if (rem != 0) {
+        MFA(oct, i, 0) += -1.0;
+        MFA(oct, i, rem) += 1.0;
+      }
+    }
+  for (oct = 0; oct < NOCT; ++oct)
+    for (im = 0; im < nm; ++im)
+      for (ia = 0; ia < na - 2; ++ia) {
+        const int rv = 37 + (im + nm * ia) % 19;
+        MFA(oct, im, ia + 0) += -1.0 * rv;
+        MFA(oct, im, ia + 1) += 2.0 * rv;
+        MFA(oct, im, ia + 2) += -1.0 * rv;
+      }
+  for (oct = 0; oct < NOCT; ++oct)
+    for (im = 0; im < nm; ++im)
+      for (ia = 0; ia < na; ++ia) {
+        MFA(oct, im, ia) /= NOCT;
+        MFA(oct, im, ia) /= sf_angle(ia);
+      }
+#undef AFM
+#undef MFA
+}
+
+/* Quantities_init_state (quantities_testing.h:60-90) looped as
+ * initialize_state (array_operations.c:26-48). */
+void oracle_init_state(OracleDims d, double* v) {
+  int ix, iy, iz, ie, im, iu;
+  for (iz = 0; iz < d.ncell_z; ++iz)
+    for (iy = 0; iy < d.ncell_y; ++iy)
+      for (ix = 0; ix < d.ncell_x; ++ix)
+        for (ie = 0; ie < d.ne; ++ie)
+          for (im = 0; im < d.nm; ++im)
+            for (iu = 0; iu < d.nu; ++iu)
+              v[ind_state(&d, ix, iy, iz, ie, im, iu)] =
+                  ((double)(1 + im)) * (double)sf_space(ix, iy, iz) *
+                  (double)sf_energy(ie) * (double)sf_unknown(iu);
+}
+
+/* Quantities_init_facexy/xz/yz (quantities_testing_kernels.h:219-332): one
+ * formula, evaluated at the out-of-domain coordinate of the face. */
+double oracle_face_value(OracleDims d, int ix_g, int iy_g, int iz_g, int ie,
+                         int ia, int iu, int octant) {
+  (void)d;
+  return ((double)(1 + ia)) * (double)sf_angle(ia) *
+         (double)sf_space(ix_g, iy_g, iz_g) * (double)sf_energy(ie) *
+         (double)sf_unknown(iu) * (double)sf_octant(octant);
+}
+
+/* Per-(octant, ie) sweep over the whole grid; faces are private to the call.
+ * Body follows sweeper_simple_c.h:118-270 with the solve of
+ * quantities_testing_kernels.h:380-446. */
+static void sweep_one(const OracleDims* d, const double* A, const double* M,
+                      const double* vi, double* vo_part, int octant, int ie,
+                      double* fxy, double* fxz, double* fyz, double* vs) {
+  const int nx = d->ncell_x, ny = d->ncell_y, nz = d->ncell_z;
+  const int na = d->na, nm = d->nm, nu = d->nu;
+  const int dx = dir_of(octant, 0), dy = dir_of(octant, 1), dz = dir_of(octant, 2);
+  int ix, iy, iz, ia, iu, im;
+  /* face layouts as ref_facexy/xz/yz with one octant and one ie:
+   * fxy[iy][ix][iu][ia], fxz[iz][ix][iu][ia], fyz[iz][iy][iu][ia] */
+#define FXY(x, y, a, u) fxy[(a) + na * ((u) + nu * ((x) + nx * (y)))]
+#define FXZ(x, z, a, u) fxz[(a) + na * ((u) + nu * ((x) + nx * (z)))]
+#define FYZ(y, z, a, u) fyz[(a) + na * ((u) + nu * ((y) + ny * (z)))]
+  /* face init "one cell outside" the domain (sweeper_simple_c.h:124-186) */
+  iz = dz == +1 ? -1 : nz;
+  for (iy = 0; iy < ny; ++iy)
+    for (ix = 0; ix < nx; ++ix)
+      for (iu = 0; iu < nu; ++iu)
+        for (ia = 0; ia < na; ++ia)
+          FXY(ix, iy, ia, iu) = oracle_face_value(*d, ix, iy, iz, ie, ia, iu, octant);
+  iy = dy == +1 ? -1 : ny;
+  for (iz = 0; iz < nz; ++iz)
+    for (ix = 0; ix < nx; ++ix)
+      for (iu = 0; iu < nu; ++iu)
+        for (ia = 0; ia < na; ++ia)
+          FXZ(ix, iz, ia, iu) = oracle_face_value(*d, ix, iy, iz, ie, ia, iu, octant);
+  ix = dx == +1 ? -1 : nx;
+  for (iz = 0; iz < nz; ++iz)
+    for (iy = 0; iy < ny; ++iy)
+      for (iu = 0; iu < nu; ++iu)
+        for (ia = 0; ia < na; ++ia)
+          FYZ(iy, iz, ia, iu) = oracle_face_value(*d, ix, iy, iz, ie, ia, iu, octant);
+
+  {
+    const int ixb = dx == +1 ? 0 : nx - 1, iyb = dy == +1 ? 0 : ny - 1,
+              izb = dz == +1 ? 0 : nz - 1;
+    const double so = (double)sf_octant(octant);
+    const double so_r = 1.0 / so;
+    int cz, cy, cx;
+    for (cz = 0, iz = izb; cz < nz; ++cz, iz += dz)
+      for (cy = 0, iy = iyb; cy < ny; ++cy, iy += dy)
+        for (cx = 0, ix = ixb; cx < nx; ++cx, ix += dx) {
+          const size_t base = ind_state(d, ix, iy, iz, ie, 0, 0);
+          const double s = (double)sf_space(ix, iy, iz);
+          const double s_r = 1.0 / s;
+          const double sx_r = 1.0 / (double)sf_space(ix - dx, iy, iz);
+          const double sy_r = 1.0 / (double)sf_space(ix, iy - dy, iz);
+          const double sz_r = 1.0 / (double)sf_space(ix, iy, iz - dz);
+          /* m2a: vs[iu][ia] = sum_im A[oct][im][ia] * vi[iu][im] (:219-232) */
+          for (iu = 0; iu < nu; ++iu)
+            for (ia = 0; ia < na; ++ia) {
+              double r = 0.0;
+              for (im = 0; im < nm; ++im)
+                r += A[ia + na * (im + nm * octant)] * vi[base + im + (size_t)nm * iu];
+              vs[ia + na * iu] = r;
+            }
+          /* solve (quantities_testing_kernels.h:421-443) */
+          for (ia = 0; ia < na; ++ia) {
+            const double wx = 1.0 / 2.0, wy = 1.0 / 4.0;
+            const double wz = 1.0 / 4.0 - 1.0 / (double)sf_angle(ia);
+            for (iu = 0; iu < nu; ++iu) {
+              const double result =
+                  (vs[ia + na * iu] * s_r +
+                   (FXY(ix, iy, ia, iu) * wx * sz_r +
+                    FXZ(ix, iz, ia, iu) * wy * sy_r +
+                    FYZ(iy, iz, ia, iu) * wz * sx_r) * so_r) * s;
+              const double scaled = result * so;
+              vs[ia + na * iu] = result;
+              FXY(ix, iy, ia, iu) = scaled;
+              FXZ(ix, iz, ia, iu) = scaled;
+              FYZ(iy, iz, ia, iu) = scaled;
+            }
+          }
+          /* a2m: vo[iu][im] += sum_ia M[oct][ia][im] * vs[iu][ia] (:256-269) */
+          for (iu = 0; iu < nu; ++iu)
+            for (im = 0; im < nm; ++im) {
+              double r = 0.0;
+              for (ia = 0; ia < na; ++ia)
+                r += M[im + nm * (ia + na * octant)] * vs[ia + na * iu];
+              vo_part[base + im + (size_t)nm * iu] += r;
+            }
+        }
+  }
+#undef FXY
+#undef FXZ
+#undef FYZ
+}
+
+void oracle_sweep_range(OracleDims d, const double* A, const double* M,
+                        const double* vi, double* vo, int ie_begin, int ie_end,
+                        int nthreads) {
+  const int nx = d.ncell_x, ny = d.ncell_y, nz = d.ncell_z;
+  const size_t nfxy = (size_t)nx * ny * d.na * d.nu;
+  const size_t nfxz = (size_t)nx * nz * d.na * d.nu;
+  const size_t nfyz = (size_t)ny * nz * d.na * d.nu;
+  int ie;
+  /* vo = 0 (sweeper_simple_c.h:106) for the groups swept */
+  {
+    int iz;
+    for (iz = 0; iz < nz; ++iz)
+      for (ie = ie_begin; ie < ie_end; ++ie)
+        memset(&vo[ind_state(&d, 0, 0, iz, ie, 0, 0)], 0,
+               sizeof(double) * (size_t)nx * ny * d.nm * d.nu);
+  }
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+  for (ie = ie_begin; ie < ie_end; ++ie) {
+    /* Energy groups are independent; octants are accumulated in the
+     * reference order 0..7 (sweeper_simple_c.h:110), so the result does not
+     * depend on the thread count. */
+    double* fxy = (double*)malloc(sizeof(double) * nfxy);
+    double* fxz = (double*)malloc(sizeof(double) * nfxz);
+    double* fyz = (double*)malloc(sizeof(double) * nfyz);
+    double* vs = (double*)malloc(sizeof(double) * (size_t)d.na * d.nu);
+    int octant;
+    for (octant = 0; octant < NOCT; ++octant)
+      sweep_one(&d, A, M, vi, vo, octant, ie, fxy, fxz, fyz, vs);
+    free(fxy);
+    free(fxz);
+    free(fyz);
+    free(vs);
+  }
+  (void)nthreads;
+}
+
+void oracle_sweep(OracleDims d, const double* A, const double* M,
+                  const double* vi, double* vo, int nthreads) {
+  oracle_sweep_range(d, A, M, vi, vo, 0, d.ne, nthreads);
+}
+
+/* get_state_norms (array_operations.c:69-110), same loop order */
+void oracle_norms(OracleDims d, const double* vi, const double* vo,
+                  double* normsq, double* normsqdiff) {
+  int ix, iy, iz, ie, im, iu;
+  double n = 0.0, nd = 0.0;
+  for (iz = 0; iz < d.ncell_z; ++iz)
+    for (iy = 0; iy < d.ncell_y; ++iy)
+      for (ix = 0; ix < d.ncell_x; ++ix)
+        for (ie = 0; ie < d.ne; ++ie)
+          for (im = 0; im < d.nm; ++im)
+            for (iu = 0; iu < d.nu; ++iu) {
+              const size_t i = ind_state(&d, ix, iy, iz, ie, im, iu);
+              const double df = vi[i] - vo[i];
+              n += vo[i] * vo[i];
+              nd += df * df;
+            }
+  *normsq = n;
+  *normsqdiff = nd;
+}

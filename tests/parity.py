@@ -1,0 +1,78 @@
+"""Shared parity helpers for the tests.
+
+Tolerances (FP64 path):
+  * per element:  |a_i - b_i| <= 1e-12 * max(|b_i|, 1e-3 * max|b|)
+    The floor is needed because the reference's own orderings (KBA blocked
+    vs simple sequential sums) already differ by ~2e-11 relative on
+    elements that cancel to near zero; normwise they agree to ~1e-16.
+  * normsq:       |n_a - n_b| <= 1e-12 * n_b
+  * manufactured inputs: vo == vi exactly, normsqdiff == 0 exactly (the
+    reference's own pass criterion, src/4_driver/runner.c:221-224).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+ELEM_TOL = 1e-12
+ELEM_FLOOR = 1e-3
+NORM_TOL = 1e-12
+
+
+def elem_err(a, b) -> float:
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    scale = np.max(np.abs(b)) if b.size else 0.0
+    if scale == 0.0:
+        return float(np.max(np.abs(a))) if a.size else 0.0
+    den = np.maximum(np.abs(b), ELEM_FLOOR * scale)
+    return float(np.max(np.abs(a - b) / den))
+
+
+def assert_close(a, b, tol=ELEM_TOL, what="vo"):
+    e = elem_err(a, b)
+    assert e <= tol, f"{what}: element error {e:.3e} > {tol:.1e}"
+
+
+def sf_energy(ie):
+    return 1 << (((ie * 1366 + 150889) % 714025) & 3)
+
+
+def sf_unknown(iu):
+    return 1 << (((iu * 741 + 66037) % 312500) & 3)
+
+
+def closed_form_normsq(ncx, ncy, ncz, ne, nm, nu=4) -> int:
+    """normsq of the manufactured problem, exact (integer arithmetic).
+
+    The sweep maps vi to vi exactly, so normsq = sum vi^2 =
+    sum_im (1+im)^2 * sum_xyz S^2 * sum_ie E^2 * sum_iu U^2 with
+    S = 2^((x+3y+7z+2)&3) (src/3_sweeper/quantities_testing_kernels.h:97-126).
+    """
+    s_m = sum((1 + im) ** 2 for im in range(nm))
+    s_x = sum(4 ** ((x + 3 * y + 7 * z + 2) & 3)
+              for x in range(ncx) for y in range(ncy) for z in range(ncz))
+    s_e = sum(sf_energy(ie) ** 2 for ie in range(ne))
+    s_u = sum(sf_unknown(iu) ** 2 for iu in range(nu))
+    return s_m * s_x * s_e * s_u
+
+
+# Known-answer configurations: the reference's tester problem shapes
+# (src/4_driver/tester.c) with their compile-time NM.
+TESTER_SHAPES = [
+    # (ncx, ncy, ncz, ne, na, nm)
+    (2, 1, 2, 3, 7, 4),     # test_serial key 0 (tester.c:72-74)
+    (5, 4, 5, 3, 7, 4),     # test_serial key 1
+    (1, 2, 1, 1, 1, 4),     # test_openmp (tester.c:126)
+    (5, 4, 5, 17, 10, 4),   # test_openmp (tester.c:140)
+    (3, 4, 2, 2, 32, 4),    # test_openmp/test_cuda subblocks (tester.c:169, 346)
+    (9, 13, 3, 5, 3, 4),    # test_openmp_tasks (tester.c:239)
+    (2, 3, 4, 20, 5, 4),    # test_cuda (tester.c:312)
+    (5, 4, 5, 7, 10, 4),    # test_mpi (tester.c:417)
+    (5, 4, 6, 7, 10, 4),    # test_mpi (tester.c:428)
+    (5, 8, 16, 9, 12, 4),   # test_mpi (tester.c:442)
+    (3, 5, 6, 2, 7, 4),     # test_mpi_cuda (tester.c:480)
+    (4, 3, 5, 11, 7, 16),   # test_variants (tester.c:533)
+    (5, 5, 5, 30, 33, 16),  # sweep defaults (runner.c:80-85), header NM
+    (5, 5, 5, 30, 33, 4),   # sweep defaults, scripts' NM=4
+    (8, 8, 8, 4, 32, 16),   # 8^3 ne4 na32 nm16 CPU configuration
+]

@@ -1,0 +1,107 @@
+"""GPU parity of the plain FP64 HIP sweep (calls libminisweep.so's C-ABI).
+
+Checked against: the compiled reference's golden vectors (tests/golden/),
+the CPU oracle on seeded non-affine inputs, and the manufactured problem's
+exact answer (vo == vi, normsq in closed form).
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import minisweep_amd as msw
+from parity import (TESTER_SHAPES, assert_close, closed_form_normsq, elem_err,
+                    NORM_TOL)
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_gpu(d, vi0, niterations, torch, device):
+    a, m = msw.init_matrices(d.nm, d.na)
+    ta = torch.from_numpy(a).to(device)
+    tm = torch.from_numpy(m).to(device)
+    vi = torch.from_numpy(np.ascontiguousarray(vi0)).to(device)
+    vo = torch.full_like(vi, float("nan"))  # the sweep must not depend on vo
+    plan = msw.Plan(d)
+    for it in range(niterations):
+        if it % 2 == 0:
+            plan.sweep(vi, vo, ta, tm)
+        else:
+            plan.sweep(vo, vi, ta, tm)
+    torch.cuda.synchronize()
+    n, nd = msw.state_norms(d, vi, vo)
+    plan.close()
+    return vi.cpu().numpy(), vo.cpu().numpy(), n, nd
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
+def test_golden_reference_vectors(path, cuda):
+    import torch
+    g = np.load(path)
+    ncx, ncy, ncz, ne, nm, na, nu, nit = (int(x) for x in g["dims"])
+    print(f"case {os.path.basename(path)}", flush=True)
+    d = msw.dims(ncx, ncy, ncz, ne, nm, na, nu)
+    od = oracle.dims(ncx, ncy, ncz, ne, nm, na, nu)
+    vi0 = g["vi_in"] if "vi_in" in g else oracle.init_state(od)
+    vi, vo, n, nd = _run_gpu(d, vi0, nit, torch, cuda)
+    if "vo" in g:
+        assert_close(vo, g["vo"], what="vo vs reference")
+        assert_close(vi, g["vi"], what="vi vs reference")
+    assert abs(n - g["normsq"]) <= NORM_TOL * g["normsq"]
+    if g["normsqdiff"] == 0:
+        assert nd == 0.0
+    else:
+        assert abs(nd - g["normsqdiff"]) <= NORM_TOL * g["normsqdiff"]
+
+
+@pytest.mark.parametrize("shape", TESTER_SHAPES, ids=[str(s) for s in TESTER_SHAPES])
+def test_manufactured_exact(shape, cuda):
+    """vo == vi bit-exactly and normsq equals the closed form."""
+    import torch
+    ncx, ncy, ncz, ne, na, nm = shape
+    print(f"case {shape}", flush=True)
+    d = msw.dims(ncx, ncy, ncz, ne, nm, na, 4)
+    vi_t = torch.empty(msw.size_state(d), dtype=torch.float64, device=cuda)
+    msw.init_state(d, vi_t)
+    vi0 = vi_t.cpu().numpy()
+    od = oracle.dims(ncx, ncy, ncz, ne, nm, na, 4)
+    np.testing.assert_array_equal(vi0, oracle.init_state(od))
+    vi, vo, n, nd = _run_gpu(d, vi0, 1, torch, cuda)
+    np.testing.assert_array_equal(vo, vi0)
+    assert nd == 0.0
+    assert n == float(closed_form_normsq(ncx, ncy, ncz, ne, nm, 4))
+
+
+RANDOM_SHAPES = [
+    # ncx, ncy, ncz, ne, nm, na, nu
+    (1, 1, 1, 1, 4, 1, 1),
+    (9, 7, 3, 2, 16, 64, 4),
+    (8, 8, 8, 2, 16, 32, 4),
+    (17, 3, 5, 1, 8, 40, 2),
+    (3, 20, 9, 3, 4, 13, 3),
+    (16, 16, 1, 1, 16, 96, 1),
+    (12, 10, 11, 2, 16, 64, 4),
+]
+
+
+@pytest.mark.parametrize("shape", RANDOM_SHAPES, ids=[str(s) for s in RANDOM_SHAPES])
+def test_random_vs_oracle(shape, cuda):
+    import torch
+    ncx, ncy, ncz, ne, nm, na, nu = shape
+    print(f"case {shape}", flush=True)
+    d = msw.dims(*shape)
+    od = oracle.dims(*shape)
+    rng = np.random.default_rng(hash(shape) & 0xFFFF)
+    base = oracle.init_state(od)
+    vi0 = base * (1.0 + 0.25 * rng.uniform(-1, 1, base.size))
+    vi, vo, n, nd = _run_gpu(d, vi0, 1, torch, cuda)
+    a, m = oracle.init_matrices(od)
+    vo_ref = oracle.sweep(od, a, m, vi0)
+    assert_close(vo, vo_ref, what="vo vs oracle")
+    n_ref, nd_ref = oracle.norms(od, vi0, vo_ref)
+    assert abs(n - n_ref) <= NORM_TOL * n_ref
+    assert abs(nd - nd_ref) <= NORM_TOL * nd_ref
