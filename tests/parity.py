@@ -1,10 +1,12 @@
 """Shared parity helpers for the tests.
 
 Tolerances (FP64 path):
-  * per element:  |a_i - b_i| <= 1e-12 * max(|b_i|, 1e-3 * max|b|)
+  * per element:  |a_i - b_i| <= 1e-12 * max(|b_i|, 1e-2 * max|b|)
     The floor is needed because the reference's own orderings (KBA blocked
     vs simple sequential sums) already differ by ~2e-11 relative on
-    elements that cancel to near zero; normwise they agree to ~1e-16.
+    elements that cancel to near zero (random inputs through 3 sweeps, each
+    amplifying by ~1e8); normwise they agree to ~1e-15.
+  * normwise:     max|a - b| <= 1e-14 * max|b|
   * normsq:       |n_a - n_b| <= 1e-12 * n_b
   * manufactured inputs: vo == vi exactly, normsqdiff == 0 exactly (the
     reference's own pass criterion, src/4_driver/runner.c:221-224).
@@ -14,7 +16,8 @@ from __future__ import annotations
 import numpy as np
 
 ELEM_TOL = 1e-12
-ELEM_FLOOR = 1e-3
+ELEM_FLOOR = 1e-2
+NORMWISE_TOL = 1e-14
 NORM_TOL = 1e-12
 
 
@@ -28,9 +31,18 @@ def elem_err(a, b) -> float:
     return float(np.max(np.abs(a - b) / den))
 
 
+def normwise_err(a, b) -> float:
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    scale = np.max(np.abs(b)) if b.size else 0.0
+    return float(np.max(np.abs(a - b)) / scale) if scale else float(np.max(np.abs(a)))
+
+
 def assert_close(a, b, tol=ELEM_TOL, what="vo"):
     e = elem_err(a, b)
     assert e <= tol, f"{what}: element error {e:.3e} > {tol:.1e}"
+    n = normwise_err(a, b)
+    assert n <= NORMWISE_TOL, f"{what}: normwise error {n:.3e} > {NORMWISE_TOL:.1e}"
 
 
 def sf_energy(ie):

@@ -1,0 +1,146 @@
+"""Pinning of the CPU oracle (runs without a GPU).
+
+The oracle is trusted only because it passes these checks:
+  * golden vectors written by the compiled reference sweeper on perturbed and
+    random (non-affine) inputs, niterations 1-3, nm 4 and 16 (tests/golden/);
+  * the manufactured problem's exact answer: vo == vi bit for bit and
+    normsq equal to the integer closed form, on the reference tester shapes;
+  * the dense-operator view on tiny grids: the sweep is affine,
+    S(v) = L v + c, so S(a u + b w) = a S(u) + b S(w) + (1 - a - b) S(0);
+  * octant independence: each octant's contribution computed alone sums to
+    the full sweep (the reference sums octants 0..7 into vo).
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from parity import TESTER_SHAPES, assert_close, closed_form_normsq, NORM_TOL
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
+def test_oracle_matches_reference_golden(path):
+    g = np.load(path)
+    ncx, ncy, ncz, ne, nm, na, nu, nit = (int(x) for x in g["dims"])
+    d = oracle.dims(ncx, ncy, ncz, ne, nm, na, nu)
+    a, m = oracle.init_matrices(d)
+    vi0 = g["vi_in"] if "vi_in" in g else oracle.init_state(d)
+    vi, vo = oracle.run(d, a, m, vi0, nit)
+    n, nd = oracle.norms(d, vi, vo)
+    if "vo" in g:
+        np.testing.assert_array_equal(vi0, g["vi_in"])
+        assert_close(vo, g["vo"], what="oracle vo vs reference")
+        assert_close(vi, g["vi"], what="oracle vi vs reference")
+    assert abs(n - g["normsq"]) <= NORM_TOL * g["normsq"]
+    if g["normsqdiff"] == 0:
+        assert nd == 0.0
+    else:
+        assert abs(nd - g["normsqdiff"]) <= NORM_TOL * g["normsqdiff"]
+
+
+SMALL_TESTER = [s for s in TESTER_SHAPES
+                if s[0] * s[1] * s[2] * s[3] * s[4] <= 40000]
+
+
+@pytest.mark.parametrize("shape", SMALL_TESTER, ids=[str(s) for s in SMALL_TESTER])
+def test_oracle_manufactured_exact(shape):
+    ncx, ncy, ncz, ne, na, nm = shape
+    d = oracle.dims(ncx, ncy, ncz, ne, nm, na, 4)
+    a, m = oracle.init_matrices(d)
+    vi = oracle.init_state(d)
+    vo = oracle.sweep(d, a, m, vi)
+    np.testing.assert_array_equal(vo, vi)
+    n, nd = oracle.norms(d, vi, vo)
+    assert nd == 0.0
+    assert n == float(closed_form_normsq(ncx, ncy, ncz, ne, nm, 4))
+
+
+def test_closed_form_matches_baseline_numbers():
+    # BASELINE known answers: 8^3 ne4 nm16 and 64^3 ne64 nm16
+    assert closed_form_normsq(8, 8, 8, 4, 16) == 188156108800
+    assert closed_form_normsq(64, 64, 64, 64, 16) == 1541374843289600
+    # sweep with no arguments, scripts' NM=4 (reference printed 6.89525100e+09)
+    assert closed_form_normsq(5, 5, 5, 30, 4) == 6895251000
+
+
+def test_oracle_flops_formula():
+    d = oracle.dims(8, 8, 8, 4, 16, 32, 4)
+    assert oracle.flops(d, 1) == 159383552.0
+    d = oracle.dims(64, 64, 64, 64, 16, 64, 4)
+    assert oracle.flops(d, 1) == 2611340115968.0
+
+
+def test_oracle_is_affine():
+    """S(a u + b w) == a S(u) + b S(w) + (1-a-b) S(0) on a tiny grid."""
+    d = oracle.dims(3, 2, 2, 2, 4, 6, 2)
+    a_, m_ = oracle.init_matrices(d)
+    rng = np.random.default_rng(3)
+    n = oracle.lib().oracle_size_state(d)
+    u, w = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    s0 = oracle.sweep(d, a_, m_, np.zeros(n))
+    su, sw = oracle.sweep(d, a_, m_, u), oracle.sweep(d, a_, m_, w)
+    al, be = 0.75, -1.5
+    lhs = oracle.sweep(d, a_, m_, al * u + be * w)
+    rhs = al * su + be * sw + (1 - al - be) * s0
+    assert np.max(np.abs(lhs - rhs)) <= 1e-9 * np.max(np.abs(rhs))
+
+
+def test_oracle_single_cell_by_hand():
+    """One cell, one angle, nm=1: the sweep reduces to scalar algebra.
+
+    With nm=na=1 the matrices are A=[[1]] (quot=2 -> A[0][0] += 2; rem=0)
+    ... computed independently here from quantities_testing_c.h:90-125 and the
+    solve of quantities_testing_kernels.h:421-434 written as a formula."""
+    nm, na, nu = 1, 1, 1
+    d = oracle.dims(1, 1, 1, 1, nm, na, nu)
+    a, m = oracle.init_matrices(d)
+    # a_from_m: quot = (0+1)//1 = 1 -> A = 1; no randvalue rows (nm-2 < 0)
+    # m_from_a: quot = (0+1)//1 = 1 -> M = 1, /8, /2^0
+    assert np.all(a == 1.0) and np.all(m == 1.0 / 8)
+    v = 3.0
+    vo = oracle.sweep(d, a, m, np.array([v]))[0]
+    expect = 0.0
+    for oct_ in range(8):
+        dx, dy, dz = [-1 if (oct_ >> b) & 1 else 1 for b in range(3)]
+        S = lambda x, y, z: 2.0 ** ((x + 3 * y + 7 * z + 2) & 3)
+        so = 1.0 + oct_
+        bx, by, bz = (-1 if dx > 0 else 1), (-1 if dy > 0 else 1), (-1 if dz > 0 else 1)
+        E = 2.0 ** (((0 * 1366 + 150889) % 714025) & 3)
+        U = 2.0 ** (((0 * 741 + 66037) % 312500) & 3)
+        face = lambda x, y, z: 1.0 * 1.0 * S(x, y, z) * E * U * so
+        fxy, fxz, fyz = face(0, 0, bz), face(0, by, 0), face(bx, 0, 0)
+        wz = 0.25 - 1.0
+        r = (v / S(0, 0, 0) + (fxy * 0.5 / S(0, 0, bz) + fxz * 0.25 / S(0, by, 0)
+                               + fyz * wz / S(bx, 0, 0)) / so) * S(0, 0, 0)
+        expect += r / 8
+    assert abs(vo - expect) <= 1e-15 * abs(expect)
+
+
+def test_oracle_octant_decomposition():
+    """The sweep equals the sum of its per-octant parts (a2m is additive)."""
+    d = oracle.dims(4, 3, 2, 2, 4, 8, 2)
+    a, m = oracle.init_matrices(d)
+    rng = np.random.default_rng(5)
+    n = oracle.lib().oracle_size_state(d)
+    vi = rng.uniform(-1, 1, n)
+    full = oracle.sweep(d, a, m, vi)
+    parts = np.zeros(n)
+    for o in range(8):
+        mo = np.zeros_like(m)
+        sl = slice(o * d.nm * d.na, (o + 1) * d.nm * d.na)
+        mo[sl] = m[sl]
+        parts += oracle.sweep(d, a, mo, vi)
+    assert np.max(np.abs(parts - full)) <= 1e-12 * np.max(np.abs(full))
+
+
+def test_oracle_thread_count_invariant():
+    d = oracle.dims(5, 4, 3, 6, 4, 9, 3)
+    a, m = oracle.init_matrices(d)
+    rng = np.random.default_rng(9)
+    vi = rng.uniform(-1, 1, oracle.lib().oracle_size_state(d))
+    np.testing.assert_array_equal(oracle.sweep(d, a, m, vi, 1),
+                                  oracle.sweep(d, a, m, vi, 4))
