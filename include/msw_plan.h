@@ -76,6 +76,15 @@ int msw_plan_sweep(MswPlan* plan, const double* vi, double* vo,
                    const double* a_from_m, const double* m_from_a,
                    void* stream);
 
+/* Per-launch timing (this library): when enabled, msw_plan_sweep brackets
+ * each sweep-kernel launch with hipEvents on the sweep's stream. */
+int msw_plan_set_timing(MswPlan* plan, int enable);
+
+/* Milliseconds of each sweep-kernel launch of the last timed sweep; waits for
+ * it to finish.  Fills up to n entries and returns how many, or the negated
+ * MSW_ERR_* code on failure. */
+int msw_plan_last_timings(MswPlan* plan, float* ms, int n);
+
 /* Release the plan.  Replaces Sweeper_destroy (sweeper_kba_c.h:292-329). */
 int msw_plan_destroy(MswPlan* plan);
 

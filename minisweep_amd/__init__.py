@@ -61,6 +61,9 @@ def lib():
         L.msw_plan_num_launches.argtypes = [vp]
         L.msw_plan_sweep.argtypes = [vp, vp, vp, vp, vp, vp]
         L.msw_plan_destroy.argtypes = [vp]
+        L.msw_plan_set_timing.argtypes = [vp, ctypes.c_int]
+        L.msw_plan_last_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_float),
+                                            ctypes.c_int]
         L.msw_state_norms.argtypes = [dims_p, vp, vp, vp, vp]
         L.msw_init_state.argtypes = [dims_p, vp, vp]
         L.msw_init_matrices_host.argtypes = [ctypes.c_int, ctypes.c_int,
@@ -136,6 +139,18 @@ class Plan:
         _check(lib().msw_plan_sweep(self._h, _dptr(vi), _dptr(vo), _dptr(a_from_m),
                                     _dptr(m_from_a), _stream_ptr(stream)),
                "msw_plan_sweep")
+
+    def set_timing(self, enable: bool = True):
+        _check(lib().msw_plan_set_timing(self._h, int(enable)), "msw_plan_set_timing")
+
+    def last_timings_ms(self):
+        """Per-launch kernel milliseconds of the last timed sweep."""
+        n = self.num_launches
+        buf = (ctypes.c_float * n)()
+        got = lib().msw_plan_last_timings(self._h, buf, n)
+        if got < 0:
+            _check(-got, "msw_plan_last_timings")
+        return [float(buf[i]) for i in range(got)]
 
     def close(self):
         if self._h:

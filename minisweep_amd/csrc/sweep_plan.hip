@@ -64,8 +64,8 @@ struct SweepParams {
   double* vo;
   const double* at;  // a_from_m transposed: [8][na][nm]
   const double* mf;  // m_from_a:            [8][na][nm]
-  double* fyz;       // x-direction faces  [8][ne][nu][ny][nz][na] (octant frame)
-  double* fxz;       // y-direction faces  [8][ne][nu][nx][nz][na] (octant frame)
+  double* fyz;       // x-direction faces  [8][ne][nu][ny][nz][na] (grid frame)
+  double* fxz;       // y-direction faces  [8][ne][nu][nx][nz][na] (grid frame)
   const TileItem* items;
   int nx, ny, nz, ne, na, nu;
   int ntx, nty;
@@ -94,11 +94,19 @@ __global__ __launch_bounds__(512) void k_sweep_fp64(SweepParams p,
   const int dz = (oct & 4) ? -1 : 1;
   const int nx = p.nx, ny = p.ny, nz = p.nz;
 
+  // Tiles are fixed in the grid's frame (cells [8g, 8g+8) along each axis),
+  // so the tiles of different octants coincide and a launch's items never
+  // overlap.  Lanes run along the octant's direction: lane xl of global tile
+  // gx is cell 8gx+xl (upward) or 8gx+7-xl (downward).  A partial last tile
+  // swept downward starts at lane xoff, the first lane inside the grid.
   const int xl = lane & (kTile - 1), yl = lane >> 3;
-  const int xp = it.tx * kTile + xl, yp = it.ty * kTile + yl;
-  const bool in_xy = xp < nx && yp < ny;
-  const int x = dx > 0 ? xp : nx - 1 - xp;
-  const int y = dy > 0 ? yp : ny - 1 - yp;
+  const int gx = dx > 0 ? it.tx : p.ntx - 1 - it.tx;
+  const int gy = dy > 0 ? it.ty : p.nty - 1 - it.ty;
+  const int x = dx > 0 ? gx * kTile + xl : gx * kTile + kTile - 1 - xl;
+  const int y = dy > 0 ? gy * kTile + yl : gy * kTile + kTile - 1 - yl;
+  const bool in_xy = x < nx && y < ny;
+  const int xoff = (dx < 0 && it.tx == 0) ? p.ntx * kTile - nx : 0;
+  const int yoff = (dy < 0 && it.ty == 0) ? p.nty * kTile - ny : 0;
 
   const int ia0 = wave * NAC;
   const int nac = min(NAC, p.na - ia0);
@@ -116,8 +124,8 @@ __global__ __launch_bounds__(512) void k_sweep_fp64(SweepParams p,
   const double eu_oct = (double)sf_energy(ie) * (double)sf_unknown(iu) * oct_s;
 
   const size_t face_ouy = (((size_t)oct * p.ne + ie) * p.nu + iu);
-  double* __restrict__ fyz = p.fyz + face_ouy * ny * nz * p.na;  // [yp][zp][ia]
-  double* __restrict__ fxz = p.fxz + face_ouy * nx * nz * p.na;  // [xp][zp][ia]
+  double* __restrict__ fyz = p.fyz + face_ouy * ny * nz * p.na;  // [y][z][ia]
+  double* __restrict__ fxz = p.fxz + face_ouy * nx * nz * p.na;  // [x][z][ia]
   const int xb = dx > 0 ? -1 : nx;  // out-of-domain coordinate of each inflow face
   const int yb = dy > 0 ? -1 : ny;
   const int zb = dz > 0 ? -1 : nz;
@@ -137,36 +145,36 @@ __global__ __launch_bounds__(512) void k_sweep_fp64(SweepParams p,
     const int z = dz > 0 ? zp : nz - 1 - zp;
 
     // ---- stage this step's x and y inflow faces in LDS --------------------
-    // Row r of lx: the x-inflow of lane (0, r), cell (yp = ty*8+r, zp = t-r).
-    // Row c of ly: the y-inflow of lane (c, 0), cell (xp = tx*8+c, zp = t-c).
+    // Row r of lx: x-inflow of lane (xoff, r), cell row y_r at zq = t-xoff-r.
+    // Row c of ly: y-inflow of lane (c, yoff), cell column x_c at zq = t-c-yoff.
     {
       const int r = lane >> 3, s = lane & 7;
       {
-        const int zq = t - r, yq = it.ty * kTile + r;
+        const int zq = t - xoff - r;
+        const int yq = dy > 0 ? gy * kTile + r : gy * kTile + kTile - 1 - r;
         if (zq >= 0 && zq < nz && yq < ny) {
+          const int zg = dz > 0 ? zq : nz - 1 - zq;
           if (it.tx == 0) {
-            const int yg = dy > 0 ? yq : ny - 1 - yq;
-            const int zg = dz > 0 ? zq : nz - 1 - zq;
-            const double sc = ldexp(eu_oct, sf_space_exp(xb, yg, zg));
+            const double sc = ldexp(eu_oct, sf_space_exp(xb, yq, zg));
             for (int i = s; i < nac; i += 8)
               lx[r * LROW + i] = (double)((1 + ia0 + i) << (i & 7)) * sc;
           } else {
-            const double* src = fyz + ((size_t)yq * nz + zq) * p.na + ia0;
+            const double* src = fyz + ((size_t)yq * nz + zg) * p.na + ia0;
             for (int i = s; i < nac; i += 8) lx[r * LROW + i] = src[i];
           }
         }
       }
       {
-        const int zq = t - r, xq = it.tx * kTile + r;
+        const int zq = t - r - yoff;
+        const int xq = dx > 0 ? gx * kTile + r : gx * kTile + kTile - 1 - r;
         if (zq >= 0 && zq < nz && xq < nx) {
+          const int zg = dz > 0 ? zq : nz - 1 - zq;
           if (it.ty == 0) {
-            const int xg = dx > 0 ? xq : nx - 1 - xq;
-            const int zg = dz > 0 ? zq : nz - 1 - zq;
-            const double sc = ldexp(eu_oct, sf_space_exp(xg, yb, zg));
+            const double sc = ldexp(eu_oct, sf_space_exp(xq, yb, zg));
             for (int i = s; i < nac; i += 8)
               ly[r * LROW + i] = (double)((1 + ia0 + i) << (i & 7)) * sc;
           } else {
-            const double* src = fxz + ((size_t)xq * nz + zq) * p.na + ia0;
+            const double* src = fxz + ((size_t)xq * nz + zg) * p.na + ia0;
             for (int i = s; i < nac; i += 8) ly[r * LROW + i] = src[i];
           }
         }
@@ -224,8 +232,8 @@ __global__ __launch_bounds__(512) void k_sweep_fp64(SweepParams p,
         for (int im = 0; im < NM; ++im) vs = fma(at_s[ia * NM + im], v[im], vs);
         const double fxn = __shfl_up(F[ia], 1);
         const double fyn = __shfl_up(F[ia], 8);
-        const double fx = xl == 0 ? lx[yl * LROW + ia] : fxn;
-        const double fy = yl == 0 ? ly[xl * LROW + ia] : fyn;
+        const double fx = xl == xoff ? lx[yl * LROW + ia] : fxn;
+        const double fy = yl == yoff ? ly[xl * LROW + ia] : fyn;
         // zfluxweight(ia) = 1/4 - 2^-(ia&7), exact; ia0 is a multiple of 8.
         const double cx = (0.25 - ldexp(1.0, -(ia & 7))) * sx_r;
         const double r = solve_fp64(vs, F[ia], fy, fx, s_r, cz, cy, cx, oct_r, s);
@@ -239,13 +247,13 @@ __global__ __launch_bounds__(512) void k_sweep_fp64(SweepParams p,
 
     // ---- outflow faces for the downstream tiles ----------------------------
     if (active && xl == kTile - 1 && it.tx + 1 < p.ntx) {
-      double* dst = fyz + ((size_t)yp * nz + zp) * p.na + ia0;
+      double* dst = fyz + ((size_t)y * nz + z) * p.na + ia0;
 #pragma unroll
       for (int i = 0; i < NAC; ++i)
         if (i < nac) dst[i] = F[i];
     }
     if (active && yl == kTile - 1 && it.ty + 1 < p.nty) {
-      double* dst = fxz + ((size_t)xp * nz + zp) * p.na + ia0;
+      double* dst = fxz + ((size_t)x * nz + z) * p.na + ia0;
 #pragma unroll
       for (int i = 0; i < NAC; ++i)
         if (i < nac) dst[i] = F[i];
@@ -445,6 +453,9 @@ struct MswPlan {
   TileItem* items;
   std::vector<int> launch_off, launch_cnt;
   size_t bytes;
+  // optional per-launch timing (hipEvents on the sweep's stream)
+  int timing;
+  std::vector<hipEvent_t> ev_start, ev_stop;
 };
 
 template <int NM, int NAC>
@@ -452,8 +463,10 @@ static int launch_all(MswPlan* p, const SweepParams& sp, hipStream_t st) {
   const dim3 block(64 * p->nwa);
   for (size_t l = 0; l < p->launch_off.size(); ++l) {
     const dim3 grid((unsigned)p->launch_cnt[l] * p->d.ne * p->d.nu);
+    if (p->timing) HIP_TRY(hipEventRecord(p->ev_start[l], st));
     hipLaunchKernelGGL((k_sweep_fp64<NM, NAC>), grid, block, p->lds_bytes, st, sp,
                        p->launch_off[l]);
+    if (p->timing) HIP_TRY(hipEventRecord(p->ev_stop[l], st));
   }
   HIP_TRY(hipGetLastError());
   return MSW_OK;
@@ -562,8 +575,43 @@ int msw_plan_num_launches(const MswPlan* p) {
   return p ? (int)p->launch_off.size() : 0;
 }
 
+int msw_plan_set_timing(MswPlan* p, int enable) {
+  if (!p) return set_err(MSW_ERR_INVALID, "null plan%s%ld");
+  if (enable && p->ev_start.empty()) {
+    const size_t n = p->launch_off.size();
+    p->ev_start.resize(n);
+    p->ev_stop.resize(n);
+    for (size_t l = 0; l < n; ++l) {
+      HIP_TRY(hipEventCreate(&p->ev_start[l]));
+      HIP_TRY(hipEventCreate(&p->ev_stop[l]));
+    }
+  }
+  p->timing = enable ? 1 : 0;
+  return MSW_OK;
+}
+
+static int last_timings(MswPlan* p, float* ms, int n, int* count) {
+  if (!p || !ms) return set_err(MSW_ERR_INVALID, "null argument%s%ld");
+  if (!p->timing || p->ev_stop.empty())
+    return set_err(MSW_ERR_INVALID, "timing is not enabled on this plan%s%ld");
+  const int m = std::min<int>(n, (int)p->ev_stop.size());
+  HIP_TRY(hipEventSynchronize(p->ev_stop.back()));
+  for (int l = 0; l < m; ++l)
+    HIP_TRY(hipEventElapsedTime(&ms[l], p->ev_start[l], p->ev_stop[l]));
+  *count = m;
+  return MSW_OK;
+}
+
+int msw_plan_last_timings(MswPlan* p, float* ms, int n) {
+  int count = 0;
+  const int rc = last_timings(p, ms, n, &count);
+  return rc == MSW_OK ? count : -rc;
+}
+
 int msw_plan_destroy(MswPlan* p) {
   if (!p) return MSW_OK;
+  for (auto e : p->ev_start) (void)hipEventDestroy(e);
+  for (auto e : p->ev_stop) (void)hipEventDestroy(e);
   if (p->at) (void)hipFree(p->at);
   if (p->mf) (void)hipFree(p->mf);
   if (p->fyz) (void)hipFree(p->fyz);
