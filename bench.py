@@ -87,9 +87,16 @@ def main():
     ap.add_argument("--na", type=int, default=64)
     ap.add_argument("--nm", type=int, default=16)
     ap.add_argument("--nu", type=int, default=4)
+    ap.add_argument("--engine", choices=["mfma", "valu"], default="mfma")
+    ap.add_argument("--angles-per-wave", type=int, default=0)
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="threads for the CPU baseline (0: OMP_NUM_THREADS or 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-gib", type=float, default=0.0,
+                    help="after the sweeps, one device-to-device copy of this many "
+                         "GiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--launch-times", action="store_true",
+                    help="print the last timed sweep's per-launch ms to stderr")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -112,7 +119,8 @@ def main():
     vi = torch.empty(size, dtype=torch.float64, device=dev)
     vo = torch.empty(size, dtype=torch.float64, device=dev)
     msw.init_state(d, vi)
-    plan = msw.Plan(d, msw.PREC_FP64)
+    engine = msw.ENGINE_MFMA if args.engine == "mfma" else msw.ENGINE_VALU
+    plan = msw.Plan(d, msw.PREC_FP64, engine=engine, angles_per_wave=args.angles_per_wave)
     stream = torch.cuda.current_stream()
 
     bufs = [vi, vo]
@@ -159,8 +167,10 @@ def main():
     traffic = read_traffic("k_sweep_fp64_64cube_traffic.txt")
     roofline = {
         "bound": "mfma",
-        "pipe": "FP64 vector FMA (MI355X FP64 vector and matrix peaks are both 78.6 TF)",
-        "kernel": "k_sweep_fp64",
+        "pipe": ("FP64 MFMA 16x16x4 for m2a/a2m, VALU for the solve" if args.engine == "mfma"
+                 else "FP64 vector FMA")
+                + " (MI355X FP64 vector and matrix peaks are both 78.6 TF)",
+        "kernel": "k_sweep_mfma_f64" if args.engine == "mfma" else "k_sweep_fp64",
         "achieved": round(achieved_tf, 3),
         "peak": PEAK_FP64_TFLOPS,
         "unit": "TFLOP/s",
@@ -195,7 +205,7 @@ def main():
             "workload": (f"plain FP64 sweep, {n}^3 cells, ne={args.ne}, na={args.na}, "
                          f"nm={args.nm}, nu={args.nu}, 8 octants, state resident in HBM"),
             "ncell": [n, n, n], "ne": args.ne, "na": args.na, "nm": args.nm,
-            "nu": args.nu, "precision": "fp64",
+            "nu": args.nu, "precision": "fp64", "engine": args.engine,
             "state_GB_each": round(size * 8 / 1e9, 3),
         },
         "latency_ms": {"p50": round(pct(50), 3), "p99": round(pct(99), 3)},
@@ -205,7 +215,19 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if args.launch_times:
+        print("launch_ms " + " ".join(f"{t:.3f}" for t in launch_ms), file=sys.stderr)
     plan.close()
+    if args.copy_probe_gib > 0:
+        del vi, vo
+        torch.cuda.empty_cache()
+        nel = int(args.copy_probe_gib * (1 << 30)) // 8
+        src = torch.ones(nel, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"copy_probe bytes={nel * 8}", file=sys.stderr)
 
 
 if __name__ == "__main__":

@@ -58,10 +58,28 @@ size_t msw_size_state(const MswDims* d);
  * (src/4_driver/runner.c:162-166). */
 double msw_flops(const MswDims* d, int niterations);
 
+/* Kernel family of a plan. */
+enum {
+  MSW_ENGINE_VALU = 0, /* contractions as per-lane FMAs, matrices in scalar registers */
+  MSW_ENGINE_MFMA = 1  /* contractions on the matrix cores (default) */
+};
+
+/* Plan options (this library). */
+typedef struct {
+  int precision;       /* MSW_PREC_* */
+  int engine;          /* MSW_ENGINE_* */
+  int angles_per_wave; /* 0: chosen from na; MFMA: 16 or 32; VALU: 8, 16 or 32 */
+} MswPlanOptions;
+
+void msw_plan_default_options(MswPlanOptions* opt);
+
 /* Create a sweep plan on the current device: schedule, face buffers and
  * the per-octant matrix staging.  Replaces the allocation half of
  * Sweeper_create (src/3_sweeper/sweeper_kba_c.h:44-287). */
 int msw_plan_create(const MswDims* d, int precision, MswPlan** plan);
+
+/* Same with explicit options (kernel engine, angles per wave). */
+int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** plan);
 
 /* Device bytes the plan holds. */
 size_t msw_plan_workspace_bytes(const MswPlan* plan);

@@ -21,6 +21,9 @@ PREC_FP64 = 0
 PREC_MIXED_FP32 = 1
 PREC_MIXED_BF16 = 2
 
+ENGINE_VALU = 0
+ENGINE_MFMA = 1
+
 NOCTANT = 8
 
 
@@ -31,6 +34,11 @@ class MswDims(ctypes.Structure):
     def tuple(self):
         return (self.ncell_x, self.ncell_y, self.ncell_z, self.ne, self.nm,
                 self.na, self.nu)
+
+
+class MswPlanOptions(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("engine", ctypes.c_int),
+                ("angles_per_wave", ctypes.c_int)]
 
 
 class SweepError(RuntimeError):
@@ -56,6 +64,9 @@ def lib():
         L.msw_flops.argtypes = [dims_p, ctypes.c_int]
         L.msw_flops.restype = ctypes.c_double
         L.msw_plan_create.argtypes = [dims_p, ctypes.c_int, ctypes.POINTER(vp)]
+        L.msw_plan_create_ex.argtypes = [dims_p, ctypes.POINTER(MswPlanOptions),
+                                         ctypes.POINTER(vp)]
+        L.msw_plan_default_options.argtypes = [ctypes.POINTER(MswPlanOptions)]
         L.msw_plan_workspace_bytes.argtypes = [vp]
         L.msw_plan_workspace_bytes.restype = ctypes.c_size_t
         L.msw_plan_num_launches.argtypes = [vp]
@@ -118,12 +129,20 @@ def _dptr(t) -> int:
 class Plan:
     """A sweep plan: schedule + face buffers for one problem shape."""
 
-    def __init__(self, d: MswDims, precision: int = PREC_FP64):
+    def __init__(self, d: MswDims, precision: int = PREC_FP64,
+                 engine: Optional[int] = None, angles_per_wave: int = 0):
         self.d = d
         self.precision = precision
+        o = MswPlanOptions()
+        lib().msw_plan_default_options(ctypes.byref(o))
+        o.precision = precision
+        if engine is not None:
+            o.engine = engine
+        o.angles_per_wave = angles_per_wave
+        self.engine = o.engine
         h = ctypes.c_void_p()
-        _check(lib().msw_plan_create(ctypes.byref(d), precision, ctypes.byref(h)),
-               "msw_plan_create")
+        _check(lib().msw_plan_create_ex(ctypes.byref(d), ctypes.byref(o), ctypes.byref(h)),
+               "msw_plan_create_ex")
         self._h = h
 
     @property

@@ -55,4 +55,22 @@ __device__ __forceinline__ float solve_fp32(float vs, float fz, float fy,
   return (vs * s_r + ((fz * cz + fy * cy) + fx * cx) * oct_r) * s;
 }
 
+// Read-only data that is uniform across a wave is loaded through the
+// constant address space, which the backend turns into scalar loads.
+typedef const __attribute__((address_space(4))) double* cdptr;
+
+// Arguments of the sweep kernels (all launches of one sweep).
+struct SweepParams {
+  const double* vi;
+  double* vo;
+  const double* at;  // a_from_m transposed: [8][na][nm]
+  const double* mf;  // m_from_a:            [8][na][nm]
+  double* fyz;       // x-direction faces  [8][ne][nu][ny][nz][na] (grid frame)
+  double* fxz;       // y-direction faces  [8][ne][nu][nx][nz][na] (grid frame)
+  const TileItem* items;
+  int nx, ny, nz, ne, na, nu;
+  int na_pad;  // na rounded up to whole angle chunks (matrix/face stride)
+  int ntx, nty;
+};
+
 }  // namespace msw

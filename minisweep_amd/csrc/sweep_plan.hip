@@ -34,6 +34,7 @@
 #include <vector>
 
 #include "msw_device.h"
+#include "msw_internal.h"
 #include "../../include/msw_plan.h"
 
 namespace msw {
@@ -55,25 +56,19 @@ static int set_err(int code, const char* fmt, const char* a = "", long b = 0) {
                      (long)__LINE__);                                        \
   } while (0)
 
-// Read-only data that is uniform across a wave is loaded through the
-// constant address space, which the backend turns into scalar loads.
-typedef const __attribute__((address_space(4))) double* cdptr;
-
-struct SweepParams {
-  const double* vi;
-  double* vo;
-  const double* at;  // a_from_m transposed: [8][na][nm]
-  const double* mf;  // m_from_a:            [8][na][nm]
-  double* fyz;       // x-direction faces  [8][ne][nu][ny][nz][na] (grid frame)
-  double* fxz;       // y-direction faces  [8][ne][nu][nx][nz][na] (grid frame)
-  const TileItem* items;
-  int nx, ny, nz, ne, na, nu;
-  int ntx, nty;
-};
-
 // ---------------------------------------------------------------------------
+// 64-bit value moved between lanes: lane reads the value of the lane whose
+// byte address (lane id * 4) it passes.
+__device__ __forceinline__ double bperm_f64(int addr, double v) {
+  const int lo = __builtin_amdgcn_ds_bpermute(addr, __double2loint(v));
+  const int hi = __builtin_amdgcn_ds_bpermute(addr, __double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+
 // FP64 tile kernel.  NM: moments (compile time), NAC: angles per wave.
-// blockDim.x = 64 * nwa, nwa = ceil(na / NAC) waves split the angles.
+// blockDim.x = 64 * nwa; wave w owns angles [w*NAC, w*NAC+NAC) of the
+// zero-padded angle range (padded angles have zero matrix rows, so they add
+// exactly nothing).
 template <int NM, int NAC>
 __global__ __launch_bounds__(512) void k_sweep_fp64(SweepParams p,
                                                     int item_base) {
@@ -93,6 +88,7 @@ __global__ __launch_bounds__(512) void k_sweep_fp64(SweepParams p,
   const int dy = (oct & 2) ? -1 : 1;
   const int dz = (oct & 4) ? -1 : 1;
   const int nx = p.nx, ny = p.ny, nz = p.nz;
+  const int nap = p.na_pad;
 
   // Tiles are fixed in the grid's frame (cells [8g, 8g+8) along each axis),
   // so the tiles of different octants coincide and a launch's items never
@@ -107,25 +103,29 @@ __global__ __launch_bounds__(512) void k_sweep_fp64(SweepParams p,
   const bool in_xy = x < nx && y < ny;
   const int xoff = (dx < 0 && it.tx == 0) ? p.ntx * kTile - nx : 0;
   const int yoff = (dy < 0 && it.ty == 0) ? p.nty * kTile - ny : 0;
+  const bool x_in = xl == xoff;  // takes its x face from the tile's inflow
+  const bool y_in = yl == yoff;
+  // byte addresses of the upwind lanes for ds_bpermute
+  const int addr_x = ((lane - 1) & 63) << 2;
+  const int addr_y = ((lane - kTile) & 63) << 2;
 
   const int ia0 = wave * NAC;
-  const int nac = min(NAC, p.na - ia0);
 
   extern __shared__ double lds[];
   double* lx = lds + (size_t)wave * (2 * kTile * LROW);  // x inflow [yl][ia]
   double* ly = lx + kTile * LROW;                       // y inflow [xl][ia]
   double* red = lds + (size_t)nwa * (2 * kTile * LROW); // [nwa][NM][64]
 
-  const double* __restrict__ at = p.at + ((size_t)oct * p.na + ia0) * NM;
-  const double* __restrict__ mf = p.mf + ((size_t)oct * p.na + ia0) * NM;
+  const double* __restrict__ at = p.at + ((size_t)oct * nap + ia0) * NM;
+  const double* __restrict__ mf = p.mf + ((size_t)oct * nap + ia0) * NM;
 
   const double oct_s = (double)(1 + oct);
   const double oct_r = 1.0 / oct_s;
   const double eu_oct = (double)sf_energy(ie) * (double)sf_unknown(iu) * oct_s;
 
   const size_t face_ouy = (((size_t)oct * p.ne + ie) * p.nu + iu);
-  double* __restrict__ fyz = p.fyz + face_ouy * ny * nz * p.na;  // [y][z][ia]
-  double* __restrict__ fxz = p.fxz + face_ouy * nx * nz * p.na;  // [x][z][ia]
+  double* __restrict__ fyz = p.fyz + face_ouy * ny * nz * nap;  // [y][z][ia]
+  double* __restrict__ fxz = p.fxz + face_ouy * nx * nz * nap;  // [x][z][ia]
   const int xb = dx > 0 ? -1 : nx;  // out-of-domain coordinate of each inflow face
   const int yb = dy > 0 ? -1 : ny;
   const int zb = dz > 0 ? -1 : nz;
@@ -144,52 +144,7 @@ __global__ __launch_bounds__(512) void k_sweep_fp64(SweepParams p,
     const bool active = in_xy && zp >= 0 && zp < nz;
     const int z = dz > 0 ? zp : nz - 1 - zp;
 
-    // ---- stage this step's x and y inflow faces in LDS --------------------
-    // Row r of lx: x-inflow of lane (xoff, r), cell row y_r at zq = t-xoff-r.
-    // Row c of ly: y-inflow of lane (c, yoff), cell column x_c at zq = t-c-yoff.
-    {
-      const int r = lane >> 3, s = lane & 7;
-      {
-        const int zq = t - xoff - r;
-        const int yq = dy > 0 ? gy * kTile + r : gy * kTile + kTile - 1 - r;
-        if (zq >= 0 && zq < nz && yq < ny) {
-          const int zg = dz > 0 ? zq : nz - 1 - zq;
-          if (it.tx == 0) {
-            const double sc = ldexp(eu_oct, sf_space_exp(xb, yq, zg));
-            for (int i = s; i < nac; i += 8)
-              lx[r * LROW + i] = (double)((1 + ia0 + i) << (i & 7)) * sc;
-          } else {
-            const double* src = fyz + ((size_t)yq * nz + zg) * p.na + ia0;
-            for (int i = s; i < nac; i += 8) lx[r * LROW + i] = src[i];
-          }
-        }
-      }
-      {
-        const int zq = t - r - yoff;
-        const int xq = dx > 0 ? gx * kTile + r : gx * kTile + kTile - 1 - r;
-        if (zq >= 0 && zq < nz && xq < nx) {
-          const int zg = dz > 0 ? zq : nz - 1 - zq;
-          if (it.ty == 0) {
-            const double sc = ldexp(eu_oct, sf_space_exp(xq, yb, zg));
-            for (int i = s; i < nac; i += 8)
-              ly[r * LROW + i] = (double)((1 + ia0 + i) << (i & 7)) * sc;
-          } else {
-            const double* src = fxz + ((size_t)xq * nz + zg) * p.na + ia0;
-            for (int i = s; i < nac; i += 8) ly[r * LROW + i] = src[i];
-          }
-        }
-      }
-    }
-    // z inflow: a lane entering the grid (zp == 0) starts from the boundary.
-    if (in_xy && zp == 0) {
-      const double sc = ldexp(eu_oct, sf_space_exp(x, y, zb));
-#pragma unroll
-      for (int i = 0; i < NAC; ++i)
-        F[i] = (double)((1 + ia0 + i) << (i & 7)) * sc;
-    }
-    __builtin_amdgcn_wave_barrier();
-
-    // ---- cell constants ----------------------------------------------------
+    // ---- vi of this step's cell: issued first, consumed by the first m2a --
     double v[NM];
     const size_t cbase = (size_t)(active ? z : 0) * sstride_z +
                          (size_t)ie * sstride_e + scell;
@@ -205,58 +160,119 @@ __global__ __launch_bounds__(512) void k_sweep_fp64(SweepParams p,
 #pragma unroll
       for (int i = 0; i < NM; ++i) v[i] = 0.0;
     }
+
+    // ---- stage this step's x and y inflow faces in LDS --------------------
+    // Row r of lx: x-inflow of lane (xoff, r), cell row y_r at zq = t-xoff-r.
+    // Row c of ly: y-inflow of lane (c, yoff), cell column x_c at zq = t-c-yoff.
+    {
+      const int r = lane >> 3, s = lane & 7;
+      {
+        const int zq = t - xoff - r;
+        const int yq = dy > 0 ? gy * kTile + r : gy * kTile + kTile - 1 - r;
+        if (zq >= 0 && zq < nz && yq < ny) {
+          const int zg = dz > 0 ? zq : nz - 1 - zq;
+          if (it.tx == 0) {
+            const double sc = ldexp(eu_oct, sf_space_exp(xb, yq, zg));
+#pragma unroll
+            for (int i = s; i < NAC; i += 8)
+              lx[r * LROW + i] = (double)((1 + ia0 + i) << (i & 7)) * sc;
+          } else {
+            const double* src = fyz + ((size_t)yq * nz + zg) * nap + ia0;
+#pragma unroll
+            for (int i = s; i < NAC; i += 8) lx[r * LROW + i] = src[i];
+          }
+        }
+      }
+      {
+        const int zq = t - r - yoff;
+        const int xq = dx > 0 ? gx * kTile + r : gx * kTile + kTile - 1 - r;
+        if (zq >= 0 && zq < nz && xq < nx) {
+          const int zg = dz > 0 ? zq : nz - 1 - zq;
+          if (it.ty == 0) {
+            const double sc = ldexp(eu_oct, sf_space_exp(xq, yb, zg));
+#pragma unroll
+            for (int i = s; i < NAC; i += 8)
+              ly[r * LROW + i] = (double)((1 + ia0 + i) << (i & 7)) * sc;
+          } else {
+            const double* src = fxz + ((size_t)xq * nz + zg) * nap + ia0;
+#pragma unroll
+            for (int i = s; i < NAC; i += 8) ly[r * LROW + i] = src[i];
+          }
+        }
+      }
+    }
+    // z inflow: a lane entering the grid (zp == 0) starts from the boundary.
+    if (in_xy && zp == 0) {
+      const double sc = ldexp(eu_oct, sf_space_exp(x, y, zb));
+#pragma unroll
+      for (int i = 0; i < NAC; ++i)
+        F[i] = (double)((1 + ia0 + i) << (i & 7)) * sc;
+    }
+    __builtin_amdgcn_wave_barrier();
+
+    // ---- cell constants ----------------------------------------------------
     const int e_s = sf_space_exp(x, y, z);
     const double s = ldexp(1.0, e_s);
     const double s_r = ldexp(1.0, -e_s);
     const double sx_r = ldexp(1.0, -sf_space_exp(x - dx, y, z));
     const double cy = ldexp(0.25, -sf_space_exp(x, y - dy, z));
     const double cz = ldexp(0.5, -sf_space_exp(x, y, z - dz));
+    const double* lxr = lx + yl * LROW;
+    const double* lyr = ly + xl * LROW;
 
     double acc[NM];
 #pragma unroll
     for (int i = 0; i < NM; ++i) acc[i] = 0.0;
 
     // The octant's matrices are read with scalar loads (constant address
-    // space: wave-uniform, through the scalar cache) every step; an opaque
-    // copy of the pointers keeps the compiler from hoisting 2*NAC*NM doubles
-    // out of the step loop into registers.
+    // space: wave-uniform, through the scalar cache).  An opaque copy of the
+    // pointers keeps the compiler from hoisting 2*NAC*NM doubles out of the
+    // step loop into registers.
     cdptr at_s = (cdptr)(at);
     cdptr mf_s = (cdptr)(mf);
     __asm__ volatile("" : "+s"(at_s), "+s"(mf_s));
 
+    // Software pipeline of the scalar matrix loads: each row pointer passes
+    // through an empty asm that takes the previous stage's result as input,
+    // so the a2m row of angle ia is issued after m2a(ia) and the m2a row of
+    // angle ia+1 after solve(ia): at most two rows (64 scalar registers)
+    // are live and each load overlaps the stage before its use.
+    cdptr arow = at_s;
 #pragma unroll
     for (int ia = 0; ia < NAC; ++ia) {
-      if (ia < nac) {
-        double vs = 0.0;
+      const double fxn = bperm_f64(addr_x, F[ia]);
+      const double fyn = bperm_f64(addr_y, F[ia]);
+      const double fxl = lxr[ia];
+      const double fyl = lyr[ia];
+      double vs = 0.0;
 #pragma unroll
-        for (int im = 0; im < NM; ++im) vs = fma(at_s[ia * NM + im], v[im], vs);
-        const double fxn = __shfl_up(F[ia], 1);
-        const double fyn = __shfl_up(F[ia], 8);
-        const double fx = xl == xoff ? lx[yl * LROW + ia] : fxn;
-        const double fy = yl == yoff ? ly[xl * LROW + ia] : fyn;
-        // zfluxweight(ia) = 1/4 - 2^-(ia&7), exact; ia0 is a multiple of 8.
-        const double cx = (0.25 - ldexp(1.0, -(ia & 7))) * sx_r;
-        const double r = solve_fp64(vs, F[ia], fy, fx, s_r, cz, cy, cx, oct_r, s);
-        F[ia] = r * oct_s;
-#pragma unroll
-        for (int im = 0; im < NM; ++im) acc[im] = fma(mf_s[ia * NM + im], r, acc[im]);
+      for (int im = 0; im < NM; ++im) vs = fma(arow[im], v[im], vs);
+      cdptr mrow = mf_s + ia * NM;
+      __asm__ volatile("" : "+s"(mrow) : "v"(vs));
+      const double fx = x_in ? fxl : fxn;
+      const double fy = y_in ? fyl : fyn;
+      // zfluxweight(ia) = 1/4 - 2^-(ia&7), exact; ia0 is a multiple of 8.
+      const double cx = (0.25 - ldexp(1.0, -(ia & 7))) * sx_r;
+      const double r = solve_fp64(vs, F[ia], fy, fx, s_r, cz, cy, cx, oct_r, s);
+      F[ia] = r * oct_s;
+      if (ia + 1 < NAC) {
+        arow = at_s + (ia + 1) * NM;
+        __asm__ volatile("" : "+s"(arow) : "v"(r));
       }
-      // keep each angle's scalar matrix loads next to their use
-      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int im = 0; im < NM; ++im) acc[im] = fma(mrow[im], r, acc[im]);
     }
 
     // ---- outflow faces for the downstream tiles ----------------------------
     if (active && xl == kTile - 1 && it.tx + 1 < p.ntx) {
-      double* dst = fyz + ((size_t)y * nz + z) * p.na + ia0;
+      double2* dst = reinterpret_cast<double2*>(fyz + ((size_t)y * nz + z) * nap + ia0);
 #pragma unroll
-      for (int i = 0; i < NAC; ++i)
-        if (i < nac) dst[i] = F[i];
+      for (int i = 0; i < NAC / 2; ++i) dst[i] = make_double2(F[2 * i], F[2 * i + 1]);
     }
     if (active && yl == kTile - 1 && it.ty + 1 < p.nty) {
-      double* dst = fxz + ((size_t)x * nz + z) * p.na + ia0;
+      double2* dst = reinterpret_cast<double2*>(fxz + ((size_t)x * nz + z) * nap + ia0);
 #pragma unroll
-      for (int i = 0; i < NAC; ++i)
-        if (i < nac) dst[i] = F[i];
+      for (int i = 0; i < NAC / 2; ++i) dst[i] = make_double2(F[2 * i], F[2 * i + 1]);
     }
 
     // ---- a2m result -> vo ----------------------------------------------------
@@ -294,15 +310,20 @@ __global__ __launch_bounds__(512) void k_sweep_fp64(SweepParams p,
   }
 }
 
-// a_from_m [8][im][ia] -> at [8][ia][im]
-__global__ void k_transpose_afm(const double* __restrict__ afm,
-                                double* __restrict__ at, int nm, int na) {
+// a_from_m [8][im][ia] -> at [8][ia_pad][im]; m_from_a [8][ia][im] -> mf
+// [8][ia_pad][im].  Rows ia >= na stay zero (set at plan creation).
+__global__ void k_pack_matrices(const double* __restrict__ afm,
+                                const double* __restrict__ mfa,
+                                double* __restrict__ at, double* __restrict__ mf,
+                                int nm, int na, int na_pad) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= kNoct * nm * na) return;
-  const int ia = i % na;
-  const int im = (i / na) % nm;
+  const int im = i % nm;
+  const int ia = (i / nm) % na;
   const int oct = i / (na * nm);
-  at[((size_t)oct * na + ia) * nm + im] = afm[i];
+  const size_t dst = ((size_t)oct * na_pad + ia) * nm + im;
+  at[dst] = afm[((size_t)oct * nm + im) * na + ia];
+  mf[dst] = mfa[i];
 }
 
 // Manufactured state v = (1+im) S(x,y,z) E(ie) U(iu), layout [iz][ie][iy][ix][iu][im].
@@ -443,7 +464,8 @@ using namespace msw;
 struct MswPlan {
   MswDims d;
   int precision;
-  int nac, nwa, ntx, nty;
+  int engine;  // MSW_ENGINE_*
+  int nac, nwa, na_pad, ntx, nty;
   size_t lds_bytes;
   double* at;  // a_from_m transposed [8][na][nm] (+ padding)
   double* mf;  // m_from_a copy       [8][na][nm] (+ padding)
@@ -464,8 +486,19 @@ static int launch_all(MswPlan* p, const SweepParams& sp, hipStream_t st) {
   for (size_t l = 0; l < p->launch_off.size(); ++l) {
     const dim3 grid((unsigned)p->launch_cnt[l] * p->d.ne * p->d.nu);
     if (p->timing) HIP_TRY(hipEventRecord(p->ev_start[l], st));
-    hipLaunchKernelGGL((k_sweep_fp64<NM, NAC>), grid, block, p->lds_bytes, st, sp,
-                       p->launch_off[l]);
+    if (p->engine == MSW_ENGINE_MFMA) {
+      LaunchSpec ls;
+      ls.item_base = p->launch_off[l];
+      ls.grid = grid.x;
+      ls.nwa = p->nwa;
+      ls.nac = p->nac;
+      ls.nm = p->d.nm;
+      ls.lds_bytes = p->lds_bytes;
+      launch_mfma_f64(ls, sp, st);
+    } else {
+      hipLaunchKernelGGL((k_sweep_fp64<NM, NAC>), grid, block, p->lds_bytes, st, sp,
+                         p->launch_off[l]);
+    }
     if (p->timing) HIP_TRY(hipEventRecord(p->ev_stop[l], st));
   }
   HIP_TRY(hipGetLastError());
@@ -513,28 +546,70 @@ static int pick_nac(int na) {
   return 32;
 }
 
+void msw_plan_default_options(MswPlanOptions* o) {
+  if (!o) return;
+  o->precision = MSW_PREC_FP64;
+  o->engine = MSW_ENGINE_MFMA;
+  o->angles_per_wave = 0;
+}
+
 int msw_plan_create(const MswDims* d, int precision, MswPlan** out) {
+  MswPlanOptions o;
+  msw_plan_default_options(&o);
+  o.precision = precision;
+  return msw_plan_create_ex(d, &o, out);
+}
+
+int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** out) {
   if (!out) return set_err(MSW_ERR_INVALID, "null plan pointer%s%ld");
   *out = nullptr;
   int rc = check_dims(d);
   if (rc) return rc;
+  if (!opt) return set_err(MSW_ERR_INVALID, "null options%s%ld");
+  const int precision = opt->precision;
   if (precision != MSW_PREC_FP64)
     return set_err(MSW_ERR_UNSUPPORTED, "precision %s%ld not built yet", "", precision);
   if (d->nm != 4 && d->nm != 8 && d->nm != 16)
     return set_err(MSW_ERR_UNSUPPORTED, "nm must be 4, 8 or 16 (got %s%ld)", "", d->nm);
+  if (opt->engine != MSW_ENGINE_VALU && opt->engine != MSW_ENGINE_MFMA)
+    return set_err(MSW_ERR_INVALID, "unknown engine %s%ld", "", opt->engine);
   MswPlan* p = new MswPlan();
   p->d = *d;
   p->precision = precision;
-  p->nac = pick_nac(d->na);
-  p->nwa = (d->na + p->nac - 1) / p->nac;
-  if (p->nwa > 8) {
-    delete p;
-    return set_err(MSW_ERR_UNSUPPORTED, "na above 256 unsupported (na=%s%ld)", "", d->na);
+  p->engine = opt->engine;
+  if (p->engine == MSW_ENGINE_MFMA) {
+    // 16-angle MFMA blocks: 16 or 32 angles per wave
+    p->nac = opt->angles_per_wave ? opt->angles_per_wave : (d->na <= 16 ? 16 : 32);
+    if (p->nac != 16 && p->nac != 32) {
+      delete p;
+      return set_err(MSW_ERR_INVALID, "MFMA engine: angles_per_wave must be 16 or 32 (got %s%ld)",
+                     "", p->nac);
+    }
+  } else {
+    p->nac = opt->angles_per_wave ? opt->angles_per_wave : pick_nac(d->na);
+    if (p->nac != 8 && p->nac != 16 && p->nac != 32) {
+      delete p;
+      return set_err(MSW_ERR_INVALID, "VALU engine: angles_per_wave must be 8, 16 or 32 (got %s%ld)",
+                     "", p->nac);
+    }
   }
-  p->ntx = (d->ncell_x + kTile - 1) / kTile;
-  p->nty = (d->ncell_y + kTile - 1) / kTile;
-  p->lds_bytes = sizeof(double) * ((size_t)p->nwa * 2 * kTile * (p->nac + 1) +
-                                   (p->nwa > 1 ? (size_t)p->nwa * d->nm * 64 : 0));
+  p->nwa = (d->na + p->nac - 1) / p->nac;
+  if (p->nwa > 4) {
+    delete p;
+    return set_err(MSW_ERR_UNSUPPORTED, "na above 4 waves of angles unsupported (na=%s%ld)",
+                   "", d->na);
+  }
+  p->na_pad = p->nwa * p->nac;
+  // tile of one wave: 8x8 cells (VALU engine), 16x4 (MFMA engine)
+  const int tx = p->engine == MSW_ENGINE_MFMA ? mfma_tile_x() : kTile;
+  const int ty = p->engine == MSW_ENGINE_MFMA ? mfma_tile_y() : kTile;
+  p->ntx = (d->ncell_x + tx - 1) / tx;
+  p->nty = (d->ncell_y + ty - 1) / ty;
+  if (p->engine == MSW_ENGINE_MFMA)
+    p->lds_bytes = mfma_f64_lds_bytes(p->nac, p->nwa);
+  else
+    p->lds_bytes = sizeof(double) * ((size_t)p->nwa * 2 * kTile * (p->nac + 1) +
+                                     (p->nwa > 1 ? (size_t)p->nwa * d->nm * 64 : 0));
   auto launches = build_schedule(p->ntx, p->nty);
   std::vector<TileItem> flat;
   for (auto& l : launches) {
@@ -542,10 +617,11 @@ int msw_plan_create(const MswDims* d, int precision, MswPlan** out) {
     p->launch_cnt.push_back((int)l.size());
     flat.insert(flat.end(), l.begin(), l.end());
   }
-  // matrix copies padded by one chunk: no scalar load can leave the buffer
-  const size_t nat = ((size_t)kNoct * d->na + 64) * d->nm;
-  const size_t nfyz = (size_t)kNoct * d->ne * d->nu * d->ncell_y * d->ncell_z * d->na;
-  const size_t nfxz = (size_t)kNoct * d->ne * d->nu * d->ncell_x * d->ncell_z * d->na;
+  // matrices in whole angle chunks (zero rows past na), plus one chunk of
+  // slack so that no scalar load can leave the buffer
+  const size_t nat = ((size_t)kNoct * p->na_pad + p->nac) * d->nm;
+  const size_t nfyz = (size_t)kNoct * d->ne * d->nu * d->ncell_y * d->ncell_z * p->na_pad;
+  const size_t nfxz = (size_t)kNoct * d->ne * d->nu * d->ncell_x * d->ncell_z * p->na_pad;
   p->bytes = sizeof(double) * (2 * nat + nfyz + nfxz + 2 * kNormBlocks) +
              sizeof(TileItem) * flat.size();
   hipError_t e = hipSuccess;
@@ -630,10 +706,8 @@ int msw_plan_sweep(MswPlan* p, const double* vi, double* vo,
   hipStream_t st = (hipStream_t)stream;
   const MswDims& d = p->d;
   const int n = kNoct * d.nm * d.na;
-  hipLaunchKernelGGL(k_transpose_afm, dim3((n + 255) / 256), dim3(256), 0, st,
-                     a_from_m, p->at, d.nm, d.na);
-  HIP_TRY(hipMemcpyAsync(p->mf, m_from_a, sizeof(double) * n,
-                         hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_pack_matrices, dim3((n + 255) / 256), dim3(256), 0, st,
+                     a_from_m, m_from_a, p->at, p->mf, d.nm, d.na, p->na_pad);
   SweepParams sp;
   sp.vi = vi;
   sp.vo = vo;
@@ -648,6 +722,7 @@ int msw_plan_sweep(MswPlan* p, const double* vi, double* vo,
   sp.ne = d.ne;
   sp.na = d.na;
   sp.nu = d.nu;
+  sp.na_pad = p->na_pad;
   sp.ntx = p->ntx;
   sp.nty = p->nty;
   switch (d.nm) {

@@ -20,13 +20,17 @@ GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.n
 pytestmark = pytest.mark.gpu
 
 
-def _run_gpu(d, vi0, niterations, torch, device):
+ENGINES = [msw.ENGINE_MFMA, msw.ENGINE_VALU]
+ENGINE_IDS = ["mfma", "valu"]
+
+
+def _run_gpu(d, vi0, niterations, torch, device, engine=msw.ENGINE_MFMA):
     a, m = msw.init_matrices(d.nm, d.na)
     ta = torch.from_numpy(a).to(device)
     tm = torch.from_numpy(m).to(device)
     vi = torch.from_numpy(np.ascontiguousarray(vi0)).to(device)
     vo = torch.full_like(vi, float("nan"))  # the sweep must not depend on vo
-    plan = msw.Plan(d)
+    plan = msw.Plan(d, engine=engine)
     for it in range(niterations):
         if it % 2 == 0:
             plan.sweep(vi, vo, ta, tm)
@@ -38,16 +42,17 @@ def _run_gpu(d, vi0, niterations, torch, device):
     return vi.cpu().numpy(), vo.cpu().numpy(), n, nd
 
 
+@pytest.mark.parametrize("engine", ENGINES, ids=ENGINE_IDS)
 @pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
-def test_golden_reference_vectors(path, cuda):
+def test_golden_reference_vectors(path, engine, cuda):
     import torch
     g = np.load(path)
     ncx, ncy, ncz, ne, nm, na, nu, nit = (int(x) for x in g["dims"])
-    print(f"case {os.path.basename(path)}", flush=True)
+    print(f"case {os.path.basename(path)} engine {engine}", flush=True)
     d = msw.dims(ncx, ncy, ncz, ne, nm, na, nu)
     od = oracle.dims(ncx, ncy, ncz, ne, nm, na, nu)
     vi0 = g["vi_in"] if "vi_in" in g else oracle.init_state(od)
-    vi, vo, n, nd = _run_gpu(d, vi0, nit, torch, cuda)
+    vi, vo, n, nd = _run_gpu(d, vi0, nit, torch, cuda, engine)
     if "vo" in g:
         # vo holds S^nit(vi0) for odd nit, S^(nit-1) for even; vi the other
         random_input = "_p2" in os.path.basename(path)
@@ -65,19 +70,20 @@ def test_golden_reference_vectors(path, cuda):
         assert abs(nd - g["normsqdiff"]) <= NORM_TOL * g["normsqdiff"]
 
 
+@pytest.mark.parametrize("engine", ENGINES, ids=ENGINE_IDS)
 @pytest.mark.parametrize("shape", TESTER_SHAPES, ids=[str(s) for s in TESTER_SHAPES])
-def test_manufactured_exact(shape, cuda):
+def test_manufactured_exact(shape, engine, cuda):
     """vo == vi bit-exactly and normsq equals the closed form."""
     import torch
     ncx, ncy, ncz, ne, na, nm = shape
-    print(f"case {shape}", flush=True)
+    print(f"case {shape} engine {engine}", flush=True)
     d = msw.dims(ncx, ncy, ncz, ne, nm, na, 4)
     vi_t = torch.empty(msw.size_state(d), dtype=torch.float64, device=cuda)
     msw.init_state(d, vi_t)
     vi0 = vi_t.cpu().numpy()
     od = oracle.dims(ncx, ncy, ncz, ne, nm, na, 4)
     np.testing.assert_array_equal(vi0, oracle.init_state(od))
-    vi, vo, n, nd = _run_gpu(d, vi0, 1, torch, cuda)
+    vi, vo, n, nd = _run_gpu(d, vi0, 1, torch, cuda, engine)
     np.testing.assert_array_equal(vo, vi0)
     assert nd == 0.0
     assert n == float(closed_form_normsq(ncx, ncy, ncz, ne, nm, 4))
@@ -95,17 +101,18 @@ RANDOM_SHAPES = [
 ]
 
 
+@pytest.mark.parametrize("engine", ENGINES, ids=ENGINE_IDS)
 @pytest.mark.parametrize("shape", RANDOM_SHAPES, ids=[str(s) for s in RANDOM_SHAPES])
-def test_random_vs_oracle(shape, cuda):
+def test_random_vs_oracle(shape, engine, cuda):
     import torch
     ncx, ncy, ncz, ne, nm, na, nu = shape
-    print(f"case {shape}", flush=True)
+    print(f"case {shape} engine {engine}", flush=True)
     d = msw.dims(*shape)
     od = oracle.dims(*shape)
     rng = np.random.default_rng(hash(shape) & 0xFFFF)
     base = oracle.init_state(od)
     vi0 = base * (1.0 + 0.25 * rng.uniform(-1, 1, base.size))
-    vi, vo, n, nd = _run_gpu(d, vi0, 1, torch, cuda)
+    vi, vo, n, nd = _run_gpu(d, vi0, 1, torch, cuda, engine)
     a, m = oracle.init_matrices(od)
     vo_ref = oracle.sweep(od, a, m, vi0)
     assert_close(vo, vo_ref, what="vo vs oracle")
