@@ -33,21 +33,9 @@ namespace msw {
 
 typedef double d4v __attribute__((ext_vector_type(4)));
 
-// DPP moves within 16-lane rows; out-of-row sources read 0.
-//   row_shl:n = 0x100 + n (lane i reads lane i+n), row_shr:n = 0x110 + n
-//   (lane i reads lane i-n).
-template <int CTRL>
-__device__ __forceinline__ int dpp_i32(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true);
-}
-template <int CTRL>
-__device__ __forceinline__ double dpp_f64(double v) {
-  return __hiloint2double(dpp_i32<CTRL>(__double2hiint(v)),
-                          dpp_i32<CTRL>(__double2loint(v)));
-}
+// DPP control for "lane i reads lane i-1 within its 16-lane row"; lanes whose
+// source is outside the row keep the update_dpp 'old' operand.
 constexpr int kRowShr1 = 0x111;
-constexpr int kRowShr8 = 0x118;
-constexpr int kRowShl8 = 0x108;
 
 // FP64 16x16x4 MFMA: A[i][k] at lane i+16k, B[k][n] at lane n+16k,
 // D[i][n] at lane n + 16*(i%4), register i/4.
@@ -79,11 +67,12 @@ __global__ __launch_bounds__(64 * NWA) void k_sweep_mfma_f64(SweepParams p,
   constexpr int NAB = NAC / 16;  // 16-angle blocks per wave
   constexpr int KM = NM / 4;     // moments per lane group in m2a
   constexpr int LROW = NAC + 1;
+  constexpr int NXIN = (kTY * NAC + 63) / 64;  // x-inflow values per lane per step
+  constexpr int NYIN = (kTX * NAC + 63) / 64;  // y-inflow values per lane per step
   static_assert(NAC % 16 == 0 && NM % 4 == 0, "block shapes");
 
   const int lane = threadIdx.x & 63;
   const int wave = NWA == 1 ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  constexpr int nwa = NWA;
   const int j = lane & 15, g = lane >> 4;
   int bid = blockIdx.x;
   const int iu = bid % p.nu;
@@ -102,6 +91,7 @@ __global__ __launch_bounds__(64 * NWA) void k_sweep_mfma_f64(SweepParams p,
   const int xoff = (dx < 0 && it.tx == 0) ? p.ntx * kTX - nx : 0;
   const int yoff = (dy < 0 && it.ty == 0) ? p.nty * kTY - ny : 0;
   const int ia0 = wave * NAC;
+  const bool first = it.first != 0;
 
   // ---- the lane's cells: (xl = j, yl = b) ----------------------------------
   const int x = dx > 0 ? gx * kTX + j : gx * kTX + kTX - 1 - j;
@@ -113,7 +103,7 @@ __global__ __launch_bounds__(64 * NWA) void k_sweep_mfma_f64(SweepParams p,
   extern __shared__ double lds[];
   double* lx = lds + (size_t)wave * ((kTX + kTY) * LROW);  // x inflow [yl][ia]
   double* ly = lx + kTY * LROW;                           // y inflow [xl][ia]
-  double* red = lds + (size_t)nwa * ((kTX + kTY) * LROW); // [nwa-1][16][64]
+  double* red = lds + (size_t)NWA * ((kTX + kTY) * LROW); // [NWA-1][16][64]
 
   const double oct_s = (double)(1 + oct);
   const double oct_r = 1.0 / oct_s;
@@ -123,8 +113,10 @@ __global__ __launch_bounds__(64 * NWA) void k_sweep_mfma_f64(SweepParams p,
   const int zb = dz > 0 ? -1 : nz;
 
   const size_t face_ouy = (((size_t)oct * p.ne + ie) * p.nu + iu);
-  double* __restrict__ fyz = p.fyz + face_ouy * ny * nz * nap;  // [y][z][ia]
-  double* __restrict__ fxz = p.fxz + face_ouy * nx * nz * nap;  // [x][z][ia]
+  const double* __restrict__ fyz_in = p.fyz + face_ouy * ny * nz * nap;  // [y][z][ia]
+  const double* __restrict__ fxz_in = p.fxz + face_ouy * nx * nz * nap;  // [x][z][ia]
+  double* fyz = p.fyz + face_ouy * ny * nz * nap;
+  double* fxz = p.fxz + face_ouy * nx * nz * nap;
 
   // ---- per-octant matrices as MFMA operands, held for the whole launch ---
   // m2a A operand, k-step s: AT[ia0 + 16a + j][KM*g + s]
@@ -158,63 +150,104 @@ __global__ __launch_bounds__(64 * NWA) void k_sweep_mfma_f64(SweepParams p,
 
   const size_t sstride_z = (size_t)p.ne * ny * nx * p.nu * NM;
   const size_t sstride_e = (size_t)ny * nx * p.nu * NM;
+  const double* __restrict__ vi_e = p.vi + (size_t)ie * sstride_e + (size_t)iu * NM;
+  double* vo_e = p.vo + (size_t)ie * sstride_e + (size_t)iu * NM;
 
-  const int nsteps = nz + kSkewM;
-  for (int t = 0; t < nsteps; ++t) {
-    // ---- m2a B operand: KM consecutive moments of the lane's cells --------
-    double viop[kTY][KM];
-    int zc[kTY];
-    bool act[kTY];
+  // ---- software pipeline: everything but the face recurrence is fetched
+  // one step ahead (vi, the old vo of the output cells, the tile inflow) --
+  double vi_n[kTY][KM];   // m2a B operand of the next step
+  double vo_n[kTY][4];    // old vo of the next step's output cells
+  double xin_n[NXIN];     // next step's x inflow (flattened [row][ia])
+  double yin_n[NYIN];     // next step's y inflow (flattened [col][ia])
+
+  auto fetch = [&](int t) {
 #pragma unroll
     for (int b = 0; b < kTY; ++b) {
       const int zp = t - j - b;
-      act[b] = x_ok && yb[b] < ny && zp >= 0 && zp < nz;
-      zc[b] = dz > 0 ? zp : nz - 1 - zp;
-      if (act[b]) {
-        const double* src = p.vi + (size_t)zc[b] * sstride_z + (size_t)ie * sstride_e +
-                            (((size_t)yb[b] * nx + x) * p.nu + iu) * NM + KM * g;
+      const bool ok = x_ok && yb[b] < ny && zp >= 0 && zp < nz;
+      const int z = dz > 0 ? zp : nz - 1 - zp;
+      const double* src = vi_e + (size_t)z * sstride_z + ((size_t)yb[b] * nx + x) * p.nu * NM +
+                          KM * g;
 #pragma unroll
-        for (int s = 0; s < KM; ++s) viop[b][s] = src[s];
-      } else {
-#pragma unroll
-        for (int s = 0; s < KM; ++s) viop[b][s] = 0.0;
-      }
+      for (int s = 0; s < KM; ++s) vi_n[b][s] = ok ? src[s] : 0.0;
     }
-
-    // ---- inflow faces of this step in LDS ------------------------------------
-    // lx row yl: x-inflow of cell (xoff, yl) at zq = t - xoff - yl
-    // ly row xl: y-inflow of cell (xl, yoff) at zq = t - xl - yoff
-    for (int idx = lane; idx < kTY * NAC; idx += 64) {
+    if (!first && j < NM) {
+#pragma unroll
+      for (int b = 0; b < kTY; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int xo_l = row_f64(g, r);
+          const int zp = t - xo_l - b;
+          const int xo = dx > 0 ? gx * kTX + xo_l : gx * kTX + kTX - 1 - xo_l;
+          const bool ok = xo < nx && yb[b] < ny && zp >= 0 && zp < nz;
+          const int zo = dz > 0 ? zp : nz - 1 - zp;
+          vo_n[b][r] = ok ? vo_e[(size_t)zo * sstride_z + ((size_t)yb[b] * nx + xo) * p.nu * NM + j]
+                          : 0.0;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NXIN; ++k) {
+      const int idx = lane + 64 * k;
       const int row = idx / NAC, i = idx % NAC;
       const int zq = t - xoff - row;
       const int yq = dy > 0 ? gy * kTY + row : gy * kTY + kTY - 1 - row;
-      if (zq >= 0 && zq < nz && yq < ny) {
-        const int zg = dz > 0 ? zq : nz - 1 - zq;
-        lx[row * LROW + i] =
-            it.tx == 0 ? (double)((1 + ia0 + i) << (i & 7)) *
-                             ldexp(eu_oct, sf_space_exp(xb, yq, zg))
-                       : fyz[((size_t)yq * nz + zg) * nap + ia0 + i];
-      }
+      const bool ok = idx < kTY * NAC && zq >= 0 && zq < nz && yq < ny;
+      const int zg = dz > 0 ? zq : nz - 1 - zq;
+      xin_n[k] = !ok ? 0.0
+                 : it.tx == 0 ? (double)((1 + ia0 + i) << (i & 7)) *
+                                    ldexp(eu_oct, sf_space_exp(xb, yq, zg))
+                              : fyz_in[((size_t)yq * nz + zg) * nap + ia0 + i];
     }
-    for (int idx = lane; idx < kTX * NAC; idx += 64) {
+#pragma unroll
+    for (int k = 0; k < NYIN; ++k) {
+      const int idx = lane + 64 * k;
       const int col = idx / NAC, i = idx % NAC;
       const int zq = t - col - yoff;
       const int xq = dx > 0 ? gx * kTX + col : gx * kTX + kTX - 1 - col;
-      if (zq >= 0 && zq < nz && xq < nx) {
-        const int zg = dz > 0 ? zq : nz - 1 - zq;
-        ly[col * LROW + i] =
-            it.ty == 0 ? (double)((1 + ia0 + i) << (i & 7)) *
-                             ldexp(eu_oct, sf_space_exp(xq, ybnd, zg))
-                       : fxz[((size_t)xq * nz + zg) * nap + ia0 + i];
-      }
+      const bool ok = idx < kTX * NAC && zq >= 0 && zq < nz && xq < nx;
+      const int zg = dz > 0 ? zq : nz - 1 - zq;
+      yin_n[k] = !ok ? 0.0
+                 : it.ty == 0 ? (double)((1 + ia0 + i) << (i & 7)) *
+                                    ldexp(eu_oct, sf_space_exp(xq, ybnd, zg))
+                              : fxz_in[((size_t)xq * nz + zg) * nap + ia0 + i];
     }
+  };
+
+  fetch(0);
+  const int nsteps = nz + kSkewM;
+  for (int t = 0; t < nsteps; ++t) {
+    // ---- take this step's prefetched data; start fetching the next -------
+    double viop[kTY][KM];
+    double voo[kTY][4];
+#pragma unroll
+    for (int b = 0; b < kTY; ++b) {
+#pragma unroll
+      for (int s = 0; s < KM; ++s) viop[b][s] = vi_n[b][s];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) voo[b][r] = vo_n[b][r];
+    }
+#pragma unroll
+    for (int k = 0; k < NXIN; ++k) {
+      const int idx = lane + 64 * k;
+      if (idx < kTY * NAC) lx[(idx / NAC) * LROW + idx % NAC] = xin_n[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NYIN; ++k) {
+      const int idx = lane + 64 * k;
+      if (idx < kTX * NAC) ly[(idx / NAC) * LROW + idx % NAC] = yin_n[k];
+    }
+    if (t + 1 < nsteps) fetch(t + 1);
 
     // ---- per cell: z inflow on entry, rescaled solve coefficients ----------
+    bool act[kTY];
+    int zc[kTY];
     double czs[kTY], cys[kTY], sxs[kTY];
 #pragma unroll
     for (int b = 0; b < kTY; ++b) {
       const int zp = t - j - b;
-      const int z = zc[b];
+      act[b] = x_ok && yb[b] < ny && zp >= 0 && zp < nz;
+      const int z = dz > 0 ? zp : nz - 1 - zp;
+      zc[b] = z;
       if (x_ok && yb[b] < ny && zp == 0) {
         const double sc = ldexp(eu_oct, sf_space_exp(x, yb[b], zb));
 #pragma unroll
@@ -231,7 +264,6 @@ __global__ __launch_bounds__(64 * NWA) void k_sweep_mfma_f64(SweepParams p,
       sxs[b] = ldexp(1.0, e_s - sf_space_exp(x - dx, yb[b], z));
     }
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_sched_barrier(0);
 
     d4v vo_acc[kTY];
 #pragma unroll
@@ -251,7 +283,7 @@ __global__ __launch_bounds__(64 * NWA) void k_sweep_mfma_f64(SweepParams p,
       // solve.  The y face of block b is block b-1's face from the previous
       // step (same lane, same register): blocks go in descending order so
       // it is read before it is overwritten.  The x face is lane j-1's
-      // (DPP row shift); lane j = 0 keeps the tile inflow loaded from LDS.
+      // (DPP row shift); lane j = 0 keeps the tile inflow read from LDS.
 #pragma unroll
       for (int b = kTY - 1; b >= 0; --b) {
 #pragma unroll
@@ -284,7 +316,6 @@ __global__ __launch_bounds__(64 * NWA) void k_sweep_mfma_f64(SweepParams p,
         for (int r = 0; r < 4; ++r)
           vo_acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(vs[b][r], mop[a][r], vo_acc[b],
                                                            0, 0, 0);
-      __builtin_amdgcn_sched_barrier(0);
     }
 
     // ---- outflow faces for the downstream tiles ----------------------------
@@ -338,9 +369,8 @@ __global__ __launch_bounds__(64 * NWA) void k_sweep_mfma_f64(SweepParams p,
           const int xo = dx > 0 ? gx * kTX + xo_l : gx * kTX + kTX - 1 - xo_l;
           if (xo < nx && yb[b] < ny && zp >= 0 && zp < nz) {
             const int zo = dz > 0 ? zp : nz - 1 - zp;
-            double* dst = p.vo + (size_t)zo * sstride_z + (size_t)ie * sstride_e +
-                          (((size_t)yb[b] * nx + xo) * p.nu + iu) * NM + j;
-            *dst = it.first ? vo_acc[b][r] : *dst + vo_acc[b][r];
+            vo_e[(size_t)zo * sstride_z + ((size_t)yb[b] * nx + xo) * p.nu * NM + j] =
+                first ? vo_acc[b][r] : voo[b][r] + vo_acc[b][r];
           }
         }
     }
@@ -375,8 +405,10 @@ template <int NM>
 static void launch_nm(const LaunchSpec& ls, const SweepParams& sp, hipStream_t st) {
   if (ls.nac == 16)
     launch_nac<NM, 16>(ls, sp, st);
-  else
+  else if (ls.nac == 32)
     launch_nac<NM, 32>(ls, sp, st);
+  else
+    launch_nac<NM, 64>(ls, sp, st);
 }
 
 void launch_mfma_f64(const LaunchSpec& ls, const SweepParams& sp, hipStream_t st) {

@@ -232,35 +232,32 @@ __global__ __launch_bounds__(512) void k_sweep_fp64(SweepParams p,
     cdptr mf_s = (cdptr)(mf);
     __asm__ volatile("" : "+s"(at_s), "+s"(mf_s));
 
-    // Software pipeline of the scalar matrix loads: each row pointer passes
-    // through an empty asm that takes the previous stage's result as input,
-    // so the a2m row of angle ia is issued after m2a(ia) and the m2a row of
-    // angle ia+1 after solve(ia): at most two rows (64 scalar registers)
-    // are live and each load overlaps the stage before its use.
-    cdptr arow = at_s;
+    // One angle at a time, its two matrix rows (m2a, a2m) read with scalar
+    // loads right before use.  The bound is an opaque copy of NAC: the
+    // uniform branch keeps the compiler from hoisting all the rows' scalar
+    // loads to the top of the step (which would exhaust scalar registers).
+    int nac_opaque = NAC;
+    __asm__ volatile("" : "+s"(nac_opaque));
 #pragma unroll
     for (int ia = 0; ia < NAC; ++ia) {
-      const double fxn = bperm_f64(addr_x, F[ia]);
-      const double fyn = bperm_f64(addr_y, F[ia]);
-      const double fxl = lxr[ia];
-      const double fyl = lyr[ia];
-      double vs = 0.0;
+      if (ia < nac_opaque) {
+        const double fxn = bperm_f64(addr_x, F[ia]);
+        const double fyn = bperm_f64(addr_y, F[ia]);
+        const double fxl = lxr[ia];
+        const double fyl = lyr[ia];
+        double vs = 0.0;
 #pragma unroll
-      for (int im = 0; im < NM; ++im) vs = fma(arow[im], v[im], vs);
-      cdptr mrow = mf_s + ia * NM;
-      __asm__ volatile("" : "+s"(mrow) : "v"(vs));
-      const double fx = x_in ? fxl : fxn;
-      const double fy = y_in ? fyl : fyn;
-      // zfluxweight(ia) = 1/4 - 2^-(ia&7), exact; ia0 is a multiple of 8.
-      const double cx = (0.25 - ldexp(1.0, -(ia & 7))) * sx_r;
-      const double r = solve_fp64(vs, F[ia], fy, fx, s_r, cz, cy, cx, oct_r, s);
-      F[ia] = r * oct_s;
-      if (ia + 1 < NAC) {
-        arow = at_s + (ia + 1) * NM;
-        __asm__ volatile("" : "+s"(arow) : "v"(r));
+        for (int im = 0; im < NM; ++im) vs = fma(at_s[ia * NM + im], v[im], vs);
+        const double fx = x_in ? fxl : fxn;
+        const double fy = y_in ? fyl : fyn;
+        // zfluxweight(ia) = 1/4 - 2^-(ia&7), exact; ia0 is a multiple of 8.
+        const double cx = (0.25 - ldexp(1.0, -(ia & 7))) * sx_r;
+        const double r = solve_fp64(vs, F[ia], fy, fx, s_r, cz, cy, cx, oct_r, s);
+        F[ia] = r * oct_s;
+#pragma unroll
+        for (int im = 0; im < NM; ++im) acc[im] = fma(mf_s[ia * NM + im], r, acc[im]);
       }
-#pragma unroll
-      for (int im = 0; im < NM; ++im) acc[im] = fma(mrow[im], r, acc[im]);
+      __builtin_amdgcn_sched_barrier(0);
     }
 
     // ---- outflow faces for the downstream tiles ----------------------------
@@ -579,11 +576,14 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
   p->engine = opt->engine;
   if (p->engine == MSW_ENGINE_MFMA) {
     // 16-angle MFMA blocks: 16 or 32 angles per wave
-    p->nac = opt->angles_per_wave ? opt->angles_per_wave : (d->na <= 16 ? 16 : 32);
-    if (p->nac != 16 && p->nac != 32) {
+    // 16 angles per wave keeps the pipelined kernel in registers; larger
+    // angle counts move to 32 or 64 so that at most 4 waves share a cell tile
+    p->nac = opt->angles_per_wave ? opt->angles_per_wave
+                                  : (d->na <= 64 ? 16 : d->na <= 128 ? 32 : 64);
+    if (p->nac != 16 && p->nac != 32 && p->nac != 64) {
       delete p;
-      return set_err(MSW_ERR_INVALID, "MFMA engine: angles_per_wave must be 16 or 32 (got %s%ld)",
-                     "", p->nac);
+      return set_err(MSW_ERR_INVALID,
+                     "MFMA engine: angles_per_wave must be 16, 32 or 64 (got %s%ld)", "", p->nac);
     }
   } else {
     p->nac = opt->angles_per_wave ? opt->angles_per_wave : pick_nac(d->na);
