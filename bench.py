@@ -88,6 +88,7 @@ def main():
     ap.add_argument("--nm", type=int, default=16)
     ap.add_argument("--nu", type=int, default=4)
     ap.add_argument("--engine", choices=["mfma", "valu"], default="mfma")
+    ap.add_argument("--precision", choices=["fp64", "fp32", "bf16"], default="fp64")
     ap.add_argument("--angles-per-wave", type=int, default=0)
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="threads for the CPU baseline (0: OMP_NUM_THREADS or 16)")
@@ -120,7 +121,9 @@ def main():
     vo = torch.empty(size, dtype=torch.float64, device=dev)
     msw.init_state(d, vi)
     engine = msw.ENGINE_MFMA if args.engine == "mfma" else msw.ENGINE_VALU
-    plan = msw.Plan(d, msw.PREC_FP64, engine=engine, angles_per_wave=args.angles_per_wave)
+    prec = {"fp64": msw.PREC_FP64, "fp32": msw.PREC_MIXED_FP32,
+            "bf16": msw.PREC_MIXED_BF16}[args.precision]
+    plan = msw.Plan(d, prec, engine=engine, angles_per_wave=args.angles_per_wave)
     stream = torch.cuda.current_stream()
 
     bufs = [vi, vo]
@@ -156,6 +159,7 @@ def main():
     normsq, normsqdiff = msw.state_norms(d, last_in, last_out)
     expect = float(closed_form_normsq(n, n, n, args.ne, args.nm, args.nu))
     exact = normsqdiff == 0.0 and normsq == expect
+    normsq_rel_err = abs(normsq - expect) / expect
 
     lat_sorted = sorted(lat)
 
@@ -205,11 +209,12 @@ def main():
             "workload": (f"plain FP64 sweep, {n}^3 cells, ne={args.ne}, na={args.na}, "
                          f"nm={args.nm}, nu={args.nu}, 8 octants, state resident in HBM"),
             "ncell": [n, n, n], "ne": args.ne, "na": args.na, "nm": args.nm,
-            "nu": args.nu, "precision": "fp64", "engine": args.engine,
+            "nu": args.nu, "precision": args.precision, "engine": args.engine,
             "state_GB_each": round(size * 8 / 1e9, 3),
         },
         "latency_ms": {"p50": round(pct(50), 3), "p99": round(pct(99), 3)},
         "exact": exact,
+        "normsq_rel_err": normsq_rel_err,
         "normsq": normsq,
         "roofline": roofline,
         "cpu_baseline": cpu,

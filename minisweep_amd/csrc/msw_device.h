@@ -65,8 +65,8 @@ struct SweepParams {
   double* vo;
   const double* at;  // a_from_m transposed: [8][na][nm]
   const double* mf;  // m_from_a:            [8][na][nm]
-  double* fyz;       // x-direction faces  [8][ne][nu][ny][nz][na] (grid frame)
-  double* fxz;       // y-direction faces  [8][ne][nu][nx][nz][na] (grid frame)
+  void* fyz;         // x-direction faces  [8][ne][nu][ny][nz][na] (grid frame)
+  void* fxz;         // y-direction faces  [8][ne][nu][nx][nz][na] (grid frame)
   const TileItem* items;
   int nx, ny, nz, ne, na, nu;
   int na_pad;  // na rounded up to whole angle chunks (matrix/face stride)

@@ -19,9 +19,12 @@ struct LaunchSpec {
   size_t lds_bytes;  // dynamic LDS per workgroup
 };
 
-// FP64 sweep with m2a/a2m on MFMA (sweep_mfma.hip).  nac must be 16 or 32.
-void launch_mfma_f64(const LaunchSpec& ls, const SweepParams& sp, hipStream_t st);
-size_t mfma_f64_lds_bytes(int nac, int nwa);
+// Sweep with m2a/a2m on MFMA (sweep_mfma.hip), in the plan's precision
+// (MSW_PREC_*).  nac is 16, 32 or 64; nwa 1..4; nm 4, 8 or 16.
+void launch_mfma(int precision, const LaunchSpec& ls, const SweepParams& sp, hipStream_t st);
+size_t mfma_lds_bytes(int precision, int nac, int nwa);
+// Bytes of one face value of the MFMA kernels in a precision.
+size_t mfma_face_elem_bytes(int precision);
 // Tile of one wave of the MFMA kernels (cells along x and y).
 int mfma_tile_x();
 int mfma_tile_y();

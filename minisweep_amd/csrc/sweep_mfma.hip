@@ -1,75 +1,173 @@
-// sweep_mfma.hip -- the sweep with m2a/a2m on the gfx950 matrix cores.
+// sweep_mfma.hip -- the sweep with m2a/a2m on the gfx950 matrix cores, in
+// FP64 (plain) and in mixed precision (FP32 or BF16 contractions with FP32
+// accumulation, FP32 faces and solve, FP64 state and vo accumulation).
 //
-// Same tiles, schedule and face buffers as the per-lane kernel in
-// sweep_plan.hip; what changes is where the contractions run and how the
-// 64 cells x NAC angles of a wave step are spread over lanes:
+// One wave owns a tile of 16 (x) by 4 (y) cells for one (octant, energy
+// group, unknown, chunk of NAC angles) and walks the whole z extent.  Lane
+// l = j + 16g owns cell (x = j, y = b) of every cell block b = 0..3; at step
+// t that cell is at octant-frame z = t - j - b, so the 64 cells of a step lie
+// on one diagonal hyperplane of the tile.
 //
-//   * The 64 cells of a step (an 8x8 tile on its skewed hyperplane) form 4
-//     cell blocks of 16 (b = 0..3: rows yl = 2b, 2b+1).  Lane l = j + 16g
-//     owns cell 16b + j of every block.
-//   * m2a: VS[angle][cell] = AT[angle][moment] . VI[moment][cell] as 16x16x4
-//     MFMAs.  AT is the A operand (per-octant, held in registers for the whole
-//     launch), VI the B operand, loaded straight from the state array with
-//     each lane reading NM/4 consecutive moments of its cell.
-//   * The MFMA result layout puts angle row(g, r) of block a, cell 16b + j in
-//     lane (j, g), register r.  The solve runs in that layout: the z face is the
-//     lane's own value from the previous step, the x face comes from lane j-1
-//     (DPP row shift by 1), the y face from lane j-8 of the same block or lane
-//     j+8 of block b-1 (DPP row shifts by 8); tile-edge inflow comes from LDS.
-//   * a2m is done transposed, VO[cell][moment] = R^T . M, so the solved values
-//     are consumed as the A operand directly from the registers they sit in
-//     (the k index of each 16x16x4 step is chosen to be the lane's own angle),
-//     and each 16-lane group stores 16 consecutive moments of one cell.
+//   * m2a: VS[angle][cell] = AT[angle][moment] . VI[moment][cell] as 16x16
+//     MFMAs.  AT is the A operand (per octant, in registers for the whole
+//     launch), VI the B operand, loaded from the state array with each lane
+//     reading consecutive moments of its own cell.
+//   * The MFMA result puts angle row(g, r) of the 16-angle block in lane
+//     (j, g), register r.  The solve runs in that layout: the z face is the
+//     lane's own value from the previous step, the y face block b-1's value of
+//     the same lane, the x face lane j-1's (DPP row shift).  Lane j = 0 gets
+//     the tile's x inflow and block 0 the y inflow, read from the face planes
+//     one step ahead.
+//   * The carried face value is the solved angular value itself; the
+//     reference's faces hold it times (1 + octant), which cancels against its
+//     1/(1 + octant), and the cell's power-of-two spatial scale is folded into
+//     the three face coefficients, so the solve is three FMAs:
+//        r = vs + Sz'*rz + Sy'*ry + Sx'*rx.
+//   * a2m: VO[moment][cell] = M^T[moment][angle] . R[angle][cell], with the
+//     solved values as the B operand straight from the registers they sit in
+//     and M^T's rows ordered so that lane (j, g) ends with moments 4g..4g+3 of
+//     cell j: vo is read and written as 32-byte runs.
+//   * The angle chunks of a tile are the waves of one workgroup; each wave
+//     stores vo for the cell blocks it owns after adding the other waves'
+//     partial sums (LDS, double-buffered by step parity, fixed order).
+//   * Face planes in HBM hold, per (octant, group, unknown), one yz plane for
+//     x faces and one xz plane for y faces, angles in lane order.  The host
+//     schedule never runs two tiles of one octant row (column) in one launch,
+//     so a tile reads its inflow and writes its outflow in place.
+//
+// MFMA layouts (gfx950):
+//   f64 16x16x4 : A[i][k] lane i+16k, B[k][n] lane n+16k, D[i][n] lane n+16(i%4) reg i/4
+//   f32 16x16x4 : A, B as f64;                           D[i][n] lane n+16(i/4) reg i%4
+//   bf16 16x16x16: A[i][k] lane i+16(k/4) elem k%4, B[k][n] lane n+16(k/4) elem k%4, D as f32
 //
 // Reference: the per-cell work is Sweeper_sweep_cell
 // (src/3_sweeper/sweeper_kba_c_kernels.h:30-555), the solve
-// Quantities_solve (src/3_sweeper/quantities_testing_kernels.h:380-446).
+// Quantities_solve (src/3_sweeper/quantities_testing_kernels.h:380-446), the
+// boundary inflow Quantities_init_face (same file, :219-332).
 #include <hip/hip_runtime.h>
 
 #include "msw_device.h"
 #include "msw_internal.h"
+#include "../../include/msw_plan.h"
 
 namespace msw {
 
 typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef short s4v __attribute__((ext_vector_type(4)));
 
 // DPP control for "lane i reads lane i-1 within its 16-lane row"; lanes whose
 // source is outside the row keep the update_dpp 'old' operand.
 constexpr int kRowShr1 = 0x111;
 
-// FP64 16x16x4 MFMA: A[i][k] at lane i+16k, B[k][n] at lane n+16k,
-// D[i][n] at lane n + 16*(i%4), register i/4.
-__device__ __forceinline__ constexpr int row_f64(int g, int r) { return g + 4 * r; }
+constexpr int kTX = 16;  // tile cells along x: one 16-lane row
+constexpr int kTY = 4;   // tile cells along y: the 4 cell blocks
 
-// The solve rescaled by the cell's power-of-two spatial factor S: with
-// czs = xfluxweight*S/Sz, cys = yfluxweight*S/Sy, cxs = zfluxweight*S/Sx,
-//   r = vs + ((fz*czs + fy*cys) + fx*cxs) * oct_r
-// rounds exactly as the reference's
-//   ((vs/S + ((fz*cz + fy*cy) + fx*cx) * oct_r) * S
-// because every rescaling is by a power of two.
-__device__ __forceinline__ double solve_scaled_f64(double vs, double fz, double fy,
-                                                   double fx, double czs,
-                                                   double cys, double cxs,
-                                                   double oct_r) {
-#pragma clang fp contract(off)
-  return vs + ((fz * czs + fy * cys) + fx * cxs) * oct_r;
+__device__ __forceinline__ double shr1_keep(double old, double v) {
+  return __hiloint2double(
+      __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(v), kRowShr1, 0xF, 0xF,
+                                  false),
+      __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(v), kRowShr1, 0xF, 0xF,
+                                  false));
+}
+__device__ __forceinline__ float shr1_keep(float old, float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v),
+                                                    kRowShr1, 0xF, 0xF, false));
 }
 
-// Tile of one wave: 16 cells along x (one 16-lane row) by 4 along y (the
-// 4 cell blocks).  Lane (j, g) owns cells (x = j, y = b) of every block b.
-constexpr int kTX = 16;
-constexpr int kTY = 4;
-constexpr int kSkewM = (kTX - 1) + (kTY - 1);
+__device__ __forceinline__ double fmaT(double a, double b, double c) {
+  return __builtin_fma(a, b, c);
+}
+__device__ __forceinline__ float fmaT(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+__device__ __forceinline__ double ldexpT(double a, int e) { return __builtin_ldexp(a, e); }
+__device__ __forceinline__ float ldexpT(float a, int e) { return __builtin_ldexpf(a, e); }
 
-template <int NM, int NAC, int NWA>
-__global__ __launch_bounds__(64 * NWA) void k_sweep_mfma_f64(SweepParams p,
-                                                             int item_base) {
-  constexpr int NAB = NAC / 16;  // 16-angle blocks per wave
-  constexpr int KM = NM / 4;     // moments per lane group in m2a
-  constexpr int LROW = NAC + 1;
-  constexpr int NXIN = (kTY * NAC + 63) / 64;  // x-inflow values per lane per step
-  constexpr int NYIN = (kTX * NAC + 63) / 64;  // y-inflow values per lane per step
-  static_assert(NAC % 16 == 0 && NM % 4 == 0, "block shapes");
+// Four consecutive values, 16-byte aligned.
+__device__ __forceinline__ void load4(const double* s, double (&v)[4]) {
+  const double2 a = reinterpret_cast<const double2*>(s)[0];
+  const double2 b = reinterpret_cast<const double2*>(s)[1];
+  v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
+__device__ __forceinline__ void load4(const float* s, float (&v)[4]) {
+  const float4 a = *reinterpret_cast<const float4*>(s);
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+__device__ __forceinline__ void store4(double* d, const double (&v)[4]) {
+  reinterpret_cast<double2*>(d)[0] = double2{v[0], v[1]};
+  reinterpret_cast<double2*>(d)[1] = double2{v[2], v[3]};
+}
+__device__ __forceinline__ void store4(float* d, const float (&v)[4]) {
+  *reinterpret_cast<float4*>(d) = float4{v[0], v[1], v[2], v[3]};
+}
+
+// ---- precision traits ------------------------------------------------------
+// T: face / solve / operand type; A: MFMA accumulator element; row(g, r): the
+// angle (within a 16-angle block) that MFMA result register r of lane group g
+// holds; mom(i): the moment that a2m result row i stands for, chosen so that
+// lane group g, register r holds moment 4g + r.
+struct PrecF64 {
+  typedef double T;
+  typedef double A;
+  typedef d4v acc4;
+  static constexpr bool kPacked = false;
+  static __device__ __forceinline__ constexpr int row(int g, int r) { return g + 4 * r; }
+  static __device__ __forceinline__ constexpr int mom(int i) { return 4 * (i & 3) + (i >> 2); }
+  static __device__ __forceinline__ acc4 mma(T a, T b, acc4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+struct PrecF32 {
+  typedef float T;
+  typedef float A;
+  typedef f4v acc4;
+  static constexpr bool kPacked = false;
+  static __device__ __forceinline__ constexpr int row(int g, int r) { return 4 * g + r; }
+  static __device__ __forceinline__ constexpr int mom(int i) { return i; }
+  static __device__ __forceinline__ acc4 mma(T a, T b, acc4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+struct PrecBF16 {
+  typedef float T;
+  typedef float A;
+  typedef f4v acc4;
+  static constexpr bool kPacked = true;  // K = 16: 4 values per lane per MFMA
+  static __device__ __forceinline__ constexpr int row(int g, int r) { return 4 * g + r; }
+  static __device__ __forceinline__ constexpr int mom(int i) { return i; }
+  static __device__ __forceinline__ acc4 mma16(const T (&a)[4], const T (&b)[4], acc4 c) {
+    s4v pa, pb;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      pa[e] = __builtin_bit_cast(short, (__bf16)a[e]);
+      pb[e] = __builtin_bit_cast(short, (__bf16)b[e]);
+    }
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(pa, pb, c, 0, 0, 0);
+  }
+};
+
+// Exponent LUT of the face coefficients.  With e = (x + 3y + 7z + 2) & 3 the
+// cell's spatial exponent, the upwind neighbours' exponents are (e + dz) & 3,
+// (e + dy) & 3 and (e - dx) & 3, so each coefficient's exponent is a function
+// of e alone: nibble e holds e - e_upwind + 8.
+__host__ __device__ __forceinline__ unsigned coef_lut(int shift) {
+  unsigned l = 0;
+  for (int e = 0; e < 4; ++e) l |= (unsigned)(e - ((e + shift) & 3) + 8) << (4 * e);
+  return l;
+}
+
+template <class PR, int NM, int NAC, int NWA>
+__global__ __launch_bounds__(64 * NWA) void k_sweep_mfma(SweepParams p, int item_base) {
+  typedef typename PR::T T;
+  typedef typename PR::A A;
+  typedef typename PR::acc4 acc4;
+  constexpr int NAB = NAC / 16;                // 16-angle blocks per wave
+  constexpr int KM = NM / 4;                   // m2a k-steps (unpacked)
+  constexpr int KA = PR::kPacked ? 4 : KM;     // m2a operand values per lane and block
+  constexpr int MSTEP = PR::kPacked ? 4 : KM;  // first m2a moment of lane group g: MSTEP*g
+  constexpr int NOWN = (kTY + NWA - 1) / NWA;  // cell blocks whose vo one wave stores
+  static_assert(NAC % 16 == 0 && NM % 4 == 0 && NM <= 16, "block shapes");
 
   const int lane = threadIdx.x & 63;
   const int wave = NWA == 1 ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -84,7 +182,7 @@ __global__ __launch_bounds__(64 * NWA) void k_sweep_mfma_f64(SweepParams p,
   const int dx = (oct & 1) ? -1 : 1;
   const int dy = (oct & 2) ? -1 : 1;
   const int dz = (oct & 4) ? -1 : 1;
-  const int nx = p.nx, ny = p.ny, nz = p.nz, nap = p.na_pad;
+  const int nx = p.nx, ny = p.ny, nz = p.nz, nu = p.nu, nap = p.na_pad;
   const int gx = dx > 0 ? it.tx : p.ntx - 1 - it.tx;
   const int gy = dy > 0 ? it.ty : p.nty - 1 - it.ty;
   // A partial last tile swept downward starts at the first lane inside.
@@ -92,336 +190,356 @@ __global__ __launch_bounds__(64 * NWA) void k_sweep_mfma_f64(SweepParams p,
   const int yoff = (dy < 0 && it.ty == 0) ? p.nty * kTY - ny : 0;
   const int ia0 = wave * NAC;
   const bool first = it.first != 0;
+  const bool x_bnd = it.tx == 0, y_bnd = it.ty == 0;
+  const bool x_out = it.tx + 1 < p.ntx, y_out = it.ty + 1 < p.nty;
 
-  // ---- the lane's cells: (xl = j, yl = b) ----------------------------------
+  // ---- the lane's cells: (x, y_b) for b = 0..3 ----------------------------
   const int x = dx > 0 ? gx * kTX + j : gx * kTX + kTX - 1 - j;
-  int yb[kTY];
-#pragma unroll
-  for (int b = 0; b < kTY; ++b) yb[b] = dy > 0 ? gy * kTY + b : gy * kTY + kTY - 1 - b;
   const bool x_ok = x < nx;
-
-  extern __shared__ double lds[];
-  double* lx = lds + (size_t)wave * ((kTX + kTY) * LROW);  // x inflow [yl][ia]
-  double* ly = lx + kTY * LROW;                           // y inflow [xl][ia]
-  double* red = lds + (size_t)NWA * ((kTX + kTY) * LROW); // [NWA-1][16][64]
-
-  const double oct_s = (double)(1 + oct);
-  const double oct_r = 1.0 / oct_s;
-  const double eu_oct = (double)sf_energy(ie) * (double)sf_unknown(iu) * oct_s;
+  const int xc = x_ok ? x : nx - 1;  // clamped: loads never leave the arrays
+  int yb[kTY], cst[kTY];
+  bool y_ok[kTY];
+  const size_t Se = (size_t)ny * nx * nu * NM;  // state stride of a group
+  const size_t Sz = (size_t)p.ne * Se;          // state stride of a z plane
+  size_t coff[kTY];                             // state offset of the cell at z = 0
+#pragma unroll
+  for (int b = 0; b < kTY; ++b) {
+    yb[b] = dy > 0 ? gy * kTY + b : gy * kTY + kTY - 1 - b;
+    y_ok[b] = yb[b] < ny;
+    const int yc = y_ok[b] ? yb[b] : ny - 1;
+    coff[b] = (size_t)ie * Se + (((size_t)yc * nx + xc) * nu + iu) * NM;
+    cst[b] = x + 3 * yb[b] + 2;
+  }
   const int xb = dx > 0 ? -1 : nx;
   const int ybnd = dy > 0 ? -1 : ny;
   const int zb = dz > 0 ? -1 : nz;
 
-  const size_t face_ouy = (((size_t)oct * p.ne + ie) * p.nu + iu);
-  const double* __restrict__ fyz_in = p.fyz + face_ouy * ny * nz * nap;  // [y][z][ia]
-  const double* __restrict__ fxz_in = p.fxz + face_ouy * nx * nz * nap;  // [x][z][ia]
-  double* fyz = p.fyz + face_ouy * ny * nz * nap;
-  double* fxz = p.fxz + face_ouy * nx * nz * nap;
+  // ---- LDS: vo partial sums [2][kTY][NWA][64][4], x inflow staging [NWA][NAB][64]
+  extern __shared__ double lds_raw[];
+  A* red = reinterpret_cast<A*>(lds_raw);
+  T* xs_lds = reinterpret_cast<T*>(red + (NWA > 1 ? 2 * kTY * NWA * 64 * 4 : 0)) +
+              (size_t)wave * NAB * 64;
 
-  // ---- per-octant matrices as MFMA operands, held for the whole launch ---
-  // m2a A operand, k-step s: AT[ia0 + 16a + j][KM*g + s]
-  double aop[NAB][KM];
-  // a2m B operand, k-step (a, r): M[ia0 + 16a + row(g, r)][moment j]
-  double mop[NAB][4];
+  // ---- face planes of this (octant, group, unknown) ------------------------
+  const size_t face_ouy = ((size_t)oct * p.ne + ie) * nu + iu;
+  T* fyz = reinterpret_cast<T*>(p.fyz) + face_ouy * ny * nz * nap + ia0 + 4 * g;  // [y][z][ia]
+  T* fxz = reinterpret_cast<T*>(p.fxz) + face_ouy * nx * nz * nap + ia0 + 4 * g;  // [x][z][ia]
+
+  // ---- per-octant operands, in registers for the whole launch --------------
+  const int moff = MSTEP * g;
+  T aop[NAB][KA];  // m2a A: AT[ia0 + 16a + j][moff + s]
+  T mop[NAB][4];   // a2m A: M[ia0 + 16a + row(g, s)][mom(j)]
+  T wz[4];         // zfluxweight of the lane's angles: 1/4 - 2^-(ia & 7)
+  T iaf[NAB][4];   // (1 + ia) 2^(ia & 7): angle factor of the boundary inflow
   {
     const double* at = p.at + ((size_t)oct * nap + ia0) * NM;
     const double* mf = p.mf + ((size_t)oct * nap + ia0) * NM;
+    const int mm = PR::mom(j);
 #pragma unroll
     for (int a = 0; a < NAB; ++a) {
 #pragma unroll
-      for (int s = 0; s < KM; ++s) aop[a][s] = at[(16 * a + j) * NM + KM * g + s];
+      for (int s = 0; s < KA; ++s) {
+        const int im = moff + s;
+        aop[a][s] = im < NM ? (T)at[(16 * a + j) * NM + (im < NM ? im : 0)] : (T)0;
+      }
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        mop[a][r] = j < NM ? mf[(16 * a + row_f64(g, r)) * NM + j] : 0.0;
+      for (int s = 0; s < 4; ++s) {
+        mop[a][s] = mm < NM ? (T)mf[(16 * a + PR::row(g, s)) * NM + (mm < NM ? mm : 0)] : (T)0;
+        const int ia = ia0 + 16 * a + PR::row(g, s);
+        iaf[a][s] = ia < p.na ? (T)((1 + ia) << (ia & 7)) : (T)0;
+      }
     }
-  }
-  // zfluxweight of the lane's angles: 1/4 - 2^-(ia & 7); ia & 7 = row & 7
-  double wz[4];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) wz[r] = 0.25 - ldexp(1.0, -(row_f64(g, r) & 7));
+    for (int s = 0; s < 4; ++s) wz[s] = (T)0.25 - ldexpT((T)1, -(PR::row(g, s) & 7));
+  }
+  const T eu = (T)(sf_energy(ie) * sf_unknown(iu));
+  const unsigned lut_z = coef_lut(dz), lut_y = coef_lut(dy), lut_x = coef_lut(-dx);
 
-  double F[NAB][kTY][4];
+  T F[NAB][kTY][4];  // outgoing face value (= solved value) of the lane's cells
 #pragma unroll
   for (int a = 0; a < NAB; ++a)
 #pragma unroll
     for (int b = 0; b < kTY; ++b)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) F[a][b][r] = 0.0;
+      for (int r = 0; r < 4; ++r) F[a][b][r] = (T)0;
 
-  const size_t sstride_z = (size_t)p.ne * ny * nx * p.nu * NM;
-  const size_t sstride_e = (size_t)ny * nx * p.nu * NM;
-  const double* __restrict__ vi_e = p.vi + (size_t)ie * sstride_e + (size_t)iu * NM;
-  double* vo_e = p.vo + (size_t)ie * sstride_e + (size_t)iu * NM;
+  // ---- data fetched one step ahead; each item is refetched for the next step
+  // right after its last use in this one -------------------------------------
+  T vin[kTY][KA];        // m2a B operand
+  double von[NOWN][4];   // old vo of the owned blocks' cells
+  T xsn[NAB];            // x inflow, spread: lane (j, g) holds block j/4, register j%4
+  T yinn[NAB][4];        // y inflow of block 0
+  const int nsteps = nz + kTX + kTY - 2;
+  const bool mpad = moff + KA > NM;  // packed operand past the last moment
 
-  // ---- software pipeline: everything but the face recurrence is fetched
-  // one step ahead (vi, the old vo of the output cells, the tile inflow) --
-  double vi_n[kTY][KM];   // m2a B operand of the next step
-  double vo_n[kTY][4];    // old vo of the next step's output cells
-  double xin_n[NXIN];     // next step's x inflow (flattened [row][ia])
-  double yin_n[NYIN];     // next step's y inflow (flattened [col][ia])
-
-  auto fetch = [&](int t) {
+  auto zclamp = [&](int zp) {
+    const int zq = zp < 0 ? 0 : zp >= nz ? nz - 1 : zp;
+    return dz > 0 ? zq : nz - 1 - zq;
+  };
+  auto fetch_vi = [&](int t) {
 #pragma unroll
     for (int b = 0; b < kTY; ++b) {
-      const int zp = t - j - b;
-      const bool ok = x_ok && yb[b] < ny && zp >= 0 && zp < nz;
-      const int z = dz > 0 ? zp : nz - 1 - zp;
-      const double* src = vi_e + (size_t)z * sstride_z + ((size_t)yb[b] * nx + x) * p.nu * NM +
-                          KM * g;
+      const double* src = p.vi + coff[b] + (size_t)zclamp(t - j - b) * Sz + moff;
+      if constexpr (KA == 4) {
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        if (!mpad) load4(src, v);
 #pragma unroll
-      for (int s = 0; s < KM; ++s) vi_n[b][s] = ok ? src[s] : 0.0;
-    }
-    if (!first && j < NM) {
+        for (int s = 0; s < 4; ++s) vin[b][s] = (T)v[s];
+      } else {
 #pragma unroll
-      for (int b = 0; b < kTY; ++b)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int xo_l = row_f64(g, r);
-          const int zp = t - xo_l - b;
-          const int xo = dx > 0 ? gx * kTX + xo_l : gx * kTX + kTX - 1 - xo_l;
-          const bool ok = xo < nx && yb[b] < ny && zp >= 0 && zp < nz;
-          const int zo = dz > 0 ? zp : nz - 1 - zp;
-          vo_n[b][r] = ok ? vo_e[(size_t)zo * sstride_z + ((size_t)yb[b] * nx + xo) * p.nu * NM + j]
-                          : 0.0;
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < NXIN; ++k) {
-      const int idx = lane + 64 * k;
-      const int row = idx / NAC, i = idx % NAC;
-      const int zq = t - xoff - row;
-      const int yq = dy > 0 ? gy * kTY + row : gy * kTY + kTY - 1 - row;
-      const bool ok = idx < kTY * NAC && zq >= 0 && zq < nz && yq < ny;
-      const int zg = dz > 0 ? zq : nz - 1 - zq;
-      xin_n[k] = !ok ? 0.0
-                 : it.tx == 0 ? (double)((1 + ia0 + i) << (i & 7)) *
-                                    ldexp(eu_oct, sf_space_exp(xb, yq, zg))
-                              : fyz_in[((size_t)yq * nz + zg) * nap + ia0 + i];
-    }
-#pragma unroll
-    for (int k = 0; k < NYIN; ++k) {
-      const int idx = lane + 64 * k;
-      const int col = idx / NAC, i = idx % NAC;
-      const int zq = t - col - yoff;
-      const int xq = dx > 0 ? gx * kTX + col : gx * kTX + kTX - 1 - col;
-      const bool ok = idx < kTX * NAC && zq >= 0 && zq < nz && xq < nx;
-      const int zg = dz > 0 ? zq : nz - 1 - zq;
-      yin_n[k] = !ok ? 0.0
-                 : it.ty == 0 ? (double)((1 + ia0 + i) << (i & 7)) *
-                                    ldexp(eu_oct, sf_space_exp(xq, ybnd, zg))
-                              : fxz_in[((size_t)xq * nz + zg) * nap + ia0 + i];
+        for (int s = 0; s < KA; ++s) vin[b][s] = (T)src[s];
+      }
     }
   };
+  auto fetch_vo = [&](int t) {
+#pragma unroll
+    for (int b = 0; b < kTY; ++b)
+      if (b % NWA == wave && 4 * g < NM)
+        load4(p.vo + coff[b] + (size_t)zclamp(t - j - b) * Sz + 4 * g, von[b / NWA]);
+  };
+  auto fetch_xs = [&](int t) {
+    const int bb = j >> 2;
+    int y = dy > 0 ? gy * kTY + bb : gy * kTY + kTY - 1 - bb;
+    y = y < ny ? y : ny - 1;
+    const T* src = fyz + ((size_t)y * nz + zclamp(t - bb)) * nap + (j & 3);
+#pragma unroll
+    for (int a = 0; a < NAB; ++a) xsn[a] = src[16 * a];
+  };
+  auto fetch_yin = [&](int t) {
+    const T* src = fxz + ((size_t)xc * nz + zclamp(t - j)) * nap;
+#pragma unroll
+    for (int a = 0; a < NAB; ++a) load4(src + 16 * a, yinn[a]);
+  };
 
-  fetch(0);
-  const int nsteps = nz + kSkewM;
+  fetch_vi(0);
+  if (!first) fetch_vo(0);
+  if (!x_bnd) fetch_xs(0);
+  if (!y_bnd) fetch_yin(0);
+
   for (int t = 0; t < nsteps; ++t) {
-    // ---- take this step's prefetched data; start fetching the next -------
-    double viop[kTY][KM];
-    double voo[kTY][4];
+    const bool more = t + 1 < nsteps;
+    // x inflow of this step into LDS: lane (0, g) reads its 16 values back
+    if (!x_bnd) {
 #pragma unroll
-    for (int b = 0; b < kTY; ++b) {
-#pragma unroll
-      for (int s = 0; s < KM; ++s) viop[b][s] = vi_n[b][s];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) voo[b][r] = vo_n[b][r];
+      for (int a = 0; a < NAB; ++a) xs_lds[a * 64 + lane] = xsn[a];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (more) fetch_xs(t + 1);
     }
-#pragma unroll
-    for (int k = 0; k < NXIN; ++k) {
-      const int idx = lane + 64 * k;
-      if (idx < kTY * NAC) lx[(idx / NAC) * LROW + idx % NAC] = xin_n[k];
-    }
-#pragma unroll
-    for (int k = 0; k < NYIN; ++k) {
-      const int idx = lane + 64 * k;
-      if (idx < kTX * NAC) ly[(idx / NAC) * LROW + idx % NAC] = yin_n[k];
-    }
-    if (t + 1 < nsteps) fetch(t + 1);
 
-    // ---- per cell: z inflow on entry, rescaled solve coefficients ----------
-    bool act[kTY];
-    int zc[kTY];
-    double czs[kTY], cys[kTY], sxs[kTY];
+    // ---- per cell block: validity, z, spatial phase; z inflow on entry -------
+    bool val[kTY];
+    int zgs[kTY], sh[kTY];
 #pragma unroll
     for (int b = 0; b < kTY; ++b) {
       const int zp = t - j - b;
-      act[b] = x_ok && yb[b] < ny && zp >= 0 && zp < nz;
-      const int z = dz > 0 ? zp : nz - 1 - zp;
-      zc[b] = z;
-      if (x_ok && yb[b] < ny && zp == 0) {
-        const double sc = ldexp(eu_oct, sf_space_exp(x, yb[b], zb));
+      val[b] = x_ok && y_ok[b] && (unsigned)zp < (unsigned)nz;
+      zgs[b] = zclamp(zp);
+      sh[b] = ((cst[b] + 7 * zgs[b]) & 3) * 4;
+      if (zp == 0) {
+        const T sc = ldexpT(eu, sf_space_exp(x, yb[b], zb));
 #pragma unroll
         for (int a = 0; a < NAB; ++a)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int ia = ia0 + 16 * a + row_f64(g, r);
-            F[a][b][r] = (double)((1 + ia) << (ia & 7)) * sc;
-          }
+          for (int r = 0; r < 4; ++r) F[a][b][r] = iaf[a][r] * sc;
       }
-      const int e_s = sf_space_exp(x, yb[b], z);
-      czs[b] = ldexp(0.5, e_s - sf_space_exp(x, yb[b], z - dz));
-      cys[b] = ldexp(0.25, e_s - sf_space_exp(x, yb[b] - dy, z));
-      sxs[b] = ldexp(1.0, e_s - sf_space_exp(x - dx, yb[b], z));
     }
-    __builtin_amdgcn_wave_barrier();
-
-    d4v vo_acc[kTY];
-#pragma unroll
-    for (int b = 0; b < kTY; ++b) vo_acc[b] = d4v{0.0, 0.0, 0.0, 0.0};
-
-#pragma unroll
-    for (int a = 0; a < NAB; ++a) {
-      // m2a for the 4 cell blocks of angle block a
-      d4v vs[kTY];
-#pragma unroll
-      for (int b = 0; b < kTY; ++b) {
-        vs[b] = d4v{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int s = 0; s < KM; ++s)
-          vs[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(aop[a][s], viop[b][s], vs[b], 0, 0, 0);
-      }
-      // solve.  The y face of block b is block b-1's face from the previous
-      // step (same lane, same register): blocks go in descending order so
-      // it is read before it is overwritten.  The x face is lane j-1's
-      // (DPP row shift); lane j = 0 keeps the tile inflow read from LDS.
-#pragma unroll
-      for (int b = kTY - 1; b >= 0; --b) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int ial = 16 * a + row_f64(g, r);
-          const double xin = lx[b * LROW + ial];
-          double fx = __hiloint2double(
-              __builtin_amdgcn_update_dpp(__double2hiint(xin), __double2hiint(F[a][b][r]),
-                                          kRowShr1, 0xF, 0xF, false),
-              __builtin_amdgcn_update_dpp(__double2loint(xin), __double2loint(F[a][b][r]),
-                                          kRowShr1, 0xF, 0xF, false));
-          if (xoff) fx = (j == xoff) ? xin : fx;
-          double fy;
-          if (b == 0) {
-            fy = ly[j * LROW + ial];
-          } else {
-            fy = F[a][b - 1][r];
-            if (yoff) fy = (b == yoff) ? ly[j * LROW + ial] : fy;
-          }
-          const double res = solve_scaled_f64(vs[b][r], F[a][b][r], fy, fx, czs[b],
-                                              cys[b], wz[r] * sxs[b], oct_r);
-          F[a][b][r] = res * oct_s;
-          vs[b][r] = res;
-        }
-      }
-      // a2m (transposed): VO[cell][moment] += R^T[cell][angle] M[angle][moment]
-#pragma unroll
-      for (int b = 0; b < kTY; ++b)
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          vo_acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(vs[b][r], mop[a][r], vo_acc[b],
-                                                           0, 0, 0);
-    }
-
-    // ---- outflow faces for the downstream tiles ----------------------------
-    if (j == kTX - 1 && it.tx + 1 < p.ntx) {
-#pragma unroll
-      for (int b = 0; b < kTY; ++b)
-        if (act[b]) {
-          double* dst = fyz + ((size_t)yb[b] * nz + zc[b]) * nap + ia0;
-#pragma unroll
-          for (int a = 0; a < NAB; ++a)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) dst[16 * a + row_f64(g, r)] = F[a][b][r];
-        }
-    }
-    if (it.ty + 1 < p.nty && act[kTY - 1]) {
-      double* dst = fxz + ((size_t)x * nz + zc[kTY - 1]) * nap + ia0;
+    // y inflow of block 0 on the domain boundary
+    if (y_bnd) {
+      const T sc = ldexpT(eu, sf_space_exp(x, ybnd, zclamp(t - j - yoff)));
 #pragma unroll
       for (int a = 0; a < NAB; ++a)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) dst[16 * a + row_f64(g, r)] = F[a][kTY - 1][r];
+        for (int r = 0; r < 4; ++r) yinn[a][r] = iaf[a][r] * sc;
     }
 
-    // ---- vo: lane (moment j, group g) holds cells x = row(g, r) of block b --
-    if constexpr (NWA > 1) {
-      if (wave > 0) {
+    acc4 vacc[kTY];
 #pragma unroll
-        for (int b = 0; b < kTY; ++b)
+    for (int b = 0; b < kTY; ++b) vacc[b] = acc4{0, 0, 0, 0};
+
 #pragma unroll
-          for (int r = 0; r < 4; ++r)
-            red[((size_t)(wave - 1) * 16 + 4 * b + r) * 64 + lane] = vo_acc[b][r];
+    for (int a = 0; a < NAB; ++a) {
+      acc4 vs[kTY];
+#pragma unroll
+      for (int b = kTY - 1; b >= 0; --b) {
+        vs[b] = acc4{0, 0, 0, 0};
+        if constexpr (PR::kPacked) {
+          vs[b] = PR::mma16(aop[a], vin[b], vs[b]);
+        } else {
+#pragma unroll
+          for (int s = 0; s < KM; ++s) vs[b] = PR::mma(aop[a][s], vin[b][s], vs[b]);
+        }
       }
-      __syncthreads();
-      if (wave == 0) {
-        // partial sums of the angle chunks, added in chunk order
+      if (a == NAB - 1 && more) fetch_vi(t + 1);
+      // solve, blocks in descending order: block b's y face is block b-1's
+      // value from the previous step, read before it is overwritten
 #pragma unroll
-        for (int w = 1; w < NWA; ++w)
+      for (int b = kTY - 1; b >= 0; --b) {
+        T xin[4];
+        if (x_bnd) {
+          const T sc = ldexpT(eu, sf_space_exp(xb, yb[b], zgs[b]));
 #pragma unroll
-          for (int b = 0; b < kTY; ++b)
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-              vo_acc[b][r] += red[((size_t)(w - 1) * 16 + 4 * b + r) * 64 + lane];
-      }
-    }
-    if (wave == 0 && j < NM) {
-#pragma unroll
-      for (int b = 0; b < kTY; ++b)
+          for (int r = 0; r < 4; ++r) xin[r] = iaf[a][r] * sc;
+        } else {
+          load4(xs_lds + a * 64 + 16 * g + 4 * b, xin);
+        }
+        const T cz = ldexpT((T)0x1p-9, (int)((lut_z >> sh[b]) & 15));
+        const T cy = ldexpT((T)0x1p-10, (int)((lut_y >> sh[b]) & 15));
+        const int xe = (int)((lut_x >> sh[b]) & 15) - 8;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int xo_l = row_f64(g, r);
-          const int zp = t - xo_l - b;
-          const int xo = dx > 0 ? gx * kTX + xo_l : gx * kTX + kTX - 1 - xo_l;
-          if (xo < nx && yb[b] < ny && zp >= 0 && zp < nz) {
-            const int zo = dz > 0 ? zp : nz - 1 - zp;
-            vo_e[(size_t)zo * sstride_z + ((size_t)yb[b] * nx + xo) * p.nu * NM + j] =
-                first ? vo_acc[b][r] : voo[b][r] + vo_acc[b][r];
+          const T rx = shr1_keep(xin[r], F[a][b][r]);
+          const T ry = b == 0 ? yinn[a][r] : F[a][b > 0 ? b - 1 : 0][r];
+          const T res = fmaT(rx, ldexpT(wz[r], xe),
+                             fmaT(ry, cy, fmaT(F[a][b][r], cz, (T)vs[b][r])));
+          F[a][b][r] = res;
+          vs[b][r] = res;
+        }
+        // Lanes left of a partial tile's first cell carry the boundary inflow
+        // so that the first cell reads it from its left neighbour next step.
+        if (x_bnd && xoff != 0 && !x_ok) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) F[a][b][r] = xin[r];
+        }
+      }
+      // Blocks below a partial tile's first row carry the y boundary inflow.
+      if (y_bnd && yoff != 0) {
+#pragma unroll
+        for (int b = 0; b < kTY - 1; ++b)
+          if (b < yoff) {
+            const T sc = ldexpT(eu, sf_space_exp(x, ybnd, zgs[b]));
+#pragma unroll
+            for (int r = 0; r < 4; ++r) F[a][b][r] = iaf[a][r] * sc;
           }
+      }
+      // a2m: VO[moment][cell] += M^T[moment][angle] R[angle][cell]
+#pragma unroll
+      for (int b = 0; b < kTY; ++b) {
+        if constexpr (PR::kPacked) {
+          T rv[4] = {(T)vs[b][0], (T)vs[b][1], (T)vs[b][2], (T)vs[b][3]};
+          vacc[b] = PR::mma16(mop[a], rv, vacc[b]);
+        } else {
+#pragma unroll
+          for (int s = 0; s < 4; ++s) vacc[b] = PR::mma(mop[a][s], (T)vs[b][s], vacc[b]);
+        }
+      }
+    }
+    if (!y_bnd && more) fetch_yin(t + 1);
+
+    // ---- outflow faces for the downstream tiles ------------------------------
+    if (x_out && j == kTX - 1) {
+#pragma unroll
+      for (int b = 0; b < kTY; ++b)
+        if (val[b]) {
+          T* dst = fyz + ((size_t)yb[b] * nz + zgs[b]) * nap;
+#pragma unroll
+          for (int a = 0; a < NAB; ++a) store4(dst + 16 * a, F[a][b]);
         }
     }
-    if constexpr (NWA > 1) __syncthreads();
+    if (y_out && val[kTY - 1]) {
+      T* dst = fxz + ((size_t)x * nz + zgs[kTY - 1]) * nap;
+#pragma unroll
+      for (int a = 0; a < NAB; ++a) store4(dst + 16 * a, F[a][kTY - 1]);
+    }
+
+    // ---- vo: lane (j, g) holds moments 4g..4g+3 of cell (j, b) ---------------
+    A* rbuf = red + (size_t)(t & 1) * kTY * NWA * 64 * 4;
+    if constexpr (NWA > 1) {
+#pragma unroll
+      for (int b = 0; b < kTY; ++b)
+        if (b % NWA != wave) {
+          A v[4] = {vacc[b][0], vacc[b][1], vacc[b][2], vacc[b][3]};
+          store4(rbuf + ((size_t)(b * NWA + wave) * 64 + lane) * 4, v);
+        }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int b = 0; b < kTY; ++b) {
+      if (b % NWA != wave) continue;
+      double sum[4];
+#pragma unroll
+      for (int w = 0; w < NWA; ++w) {
+        A v[4];
+        if (w == wave) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) v[r] = vacc[b][r];
+        } else {
+          load4(rbuf + ((size_t)(b * NWA + w) * 64 + lane) * 4, v);
+        }
+        // angle chunks added in chunk order
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sum[r] = w == 0 ? (double)v[r] : sum[r] + (double)v[r];
+      }
+      if (!first) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sum[r] += von[b / NWA][r];
+      }
+      if (val[b] && 4 * g < NM) store4(p.vo + coff[b] + (size_t)zgs[b] * Sz + 4 * g, sum);
+    }
+    if (!first && more) fetch_vo(t + 1);
   }
 }
 
-template <int NM, int NAC>
+template <class PR, int NM, int NAC>
 static void launch_nac(const LaunchSpec& ls, const SweepParams& sp, hipStream_t st) {
   const dim3 block(64 * ls.nwa);
   switch (ls.nwa) {
     case 1:
-      hipLaunchKernelGGL((k_sweep_mfma_f64<NM, NAC, 1>), dim3(ls.grid), block, ls.lds_bytes,
+      hipLaunchKernelGGL((k_sweep_mfma<PR, NM, NAC, 1>), dim3(ls.grid), block, ls.lds_bytes,
                          st, sp, ls.item_base);
       break;
     case 2:
-      hipLaunchKernelGGL((k_sweep_mfma_f64<NM, NAC, 2>), dim3(ls.grid), block, ls.lds_bytes,
+      hipLaunchKernelGGL((k_sweep_mfma<PR, NM, NAC, 2>), dim3(ls.grid), block, ls.lds_bytes,
                          st, sp, ls.item_base);
       break;
     case 3:
-      hipLaunchKernelGGL((k_sweep_mfma_f64<NM, NAC, 3>), dim3(ls.grid), block, ls.lds_bytes,
+      hipLaunchKernelGGL((k_sweep_mfma<PR, NM, NAC, 3>), dim3(ls.grid), block, ls.lds_bytes,
                          st, sp, ls.item_base);
       break;
     default:
-      hipLaunchKernelGGL((k_sweep_mfma_f64<NM, NAC, 4>), dim3(ls.grid), block, ls.lds_bytes,
+      hipLaunchKernelGGL((k_sweep_mfma<PR, NM, NAC, 4>), dim3(ls.grid), block, ls.lds_bytes,
                          st, sp, ls.item_base);
       break;
   }
 }
 
-template <int NM>
+template <class PR, int NM>
 static void launch_nm(const LaunchSpec& ls, const SweepParams& sp, hipStream_t st) {
   if (ls.nac == 16)
-    launch_nac<NM, 16>(ls, sp, st);
+    launch_nac<PR, NM, 16>(ls, sp, st);
   else if (ls.nac == 32)
-    launch_nac<NM, 32>(ls, sp, st);
+    launch_nac<PR, NM, 32>(ls, sp, st);
   else
-    launch_nac<NM, 64>(ls, sp, st);
+    launch_nac<PR, NM, 64>(ls, sp, st);
 }
 
-void launch_mfma_f64(const LaunchSpec& ls, const SweepParams& sp, hipStream_t st) {
+template <class PR>
+static void launch_prec(const LaunchSpec& ls, const SweepParams& sp, hipStream_t st) {
   switch (ls.nm) {
-    case 4: launch_nm<4>(ls, sp, st); break;
-    case 8: launch_nm<8>(ls, sp, st); break;
-    default: launch_nm<16>(ls, sp, st); break;
+    case 4: launch_nm<PR, 4>(ls, sp, st); break;
+    case 8: launch_nm<PR, 8>(ls, sp, st); break;
+    default: launch_nm<PR, 16>(ls, sp, st); break;
   }
 }
 
-size_t mfma_f64_lds_bytes(int nac, int nwa) {
-  return sizeof(double) *
-         ((size_t)nwa * (kTX + kTY) * (nac + 1) + (size_t)(nwa - 1) * 16 * 64);
+void launch_mfma(int precision, const LaunchSpec& ls, const SweepParams& sp, hipStream_t st) {
+  if (precision == MSW_PREC_MIXED_FP32)
+    launch_prec<PrecF32>(ls, sp, st);
+  else if (precision == MSW_PREC_MIXED_BF16)
+    launch_prec<PrecBF16>(ls, sp, st);
+  else
+    launch_prec<PrecF64>(ls, sp, st);
+}
+
+size_t mfma_face_elem_bytes(int precision) {
+  return precision == MSW_PREC_FP64 ? sizeof(double) : sizeof(float);
+}
+
+size_t mfma_lds_bytes(int precision, int nac, int nwa) {
+  const size_t t = mfma_face_elem_bytes(precision);  // accumulator = face type
+  const size_t red = nwa > 1 ? 2 * (size_t)kTY * nwa * 64 * 4 * t : 0;
+  return red + (size_t)nwa * (nac / 16) * 64 * t;
 }
 
 int mfma_tile_x() { return kTX; }

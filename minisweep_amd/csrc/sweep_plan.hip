@@ -124,8 +124,8 @@ __global__ __launch_bounds__(512) void k_sweep_fp64(SweepParams p,
   const double eu_oct = (double)sf_energy(ie) * (double)sf_unknown(iu) * oct_s;
 
   const size_t face_ouy = (((size_t)oct * p.ne + ie) * p.nu + iu);
-  double* __restrict__ fyz = p.fyz + face_ouy * ny * nz * nap;  // [y][z][ia]
-  double* __restrict__ fxz = p.fxz + face_ouy * nx * nz * nap;  // [x][z][ia]
+  double* __restrict__ fyz = (double*)p.fyz + face_ouy * ny * nz * nap;  // [y][z][ia]
+  double* __restrict__ fxz = (double*)p.fxz + face_ouy * nx * nz * nap;  // [x][z][ia]
   const int xb = dx > 0 ? -1 : nx;  // out-of-domain coordinate of each inflow face
   const int yb = dy > 0 ? -1 : ny;
   const int zb = dz > 0 ? -1 : nz;
@@ -466,8 +466,8 @@ struct MswPlan {
   size_t lds_bytes;
   double* at;  // a_from_m transposed [8][na][nm] (+ padding)
   double* mf;  // m_from_a copy       [8][na][nm] (+ padding)
-  double* fyz;
-  double* fxz;
+  void* fyz;   // face planes (element type of the precision)
+  void* fxz;
   double* norm_part;
   TileItem* items;
   std::vector<int> launch_off, launch_cnt;
@@ -491,7 +491,7 @@ static int launch_all(MswPlan* p, const SweepParams& sp, hipStream_t st) {
       ls.nac = p->nac;
       ls.nm = p->d.nm;
       ls.lds_bytes = p->lds_bytes;
-      launch_mfma_f64(ls, sp, st);
+      launch_mfma(p->precision, ls, sp, st);
     } else {
       hipLaunchKernelGGL((k_sweep_fp64<NM, NAC>), grid, block, p->lds_bytes, st, sp,
                          p->launch_off[l]);
@@ -564,8 +564,13 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
   if (rc) return rc;
   if (!opt) return set_err(MSW_ERR_INVALID, "null options%s%ld");
   const int precision = opt->precision;
-  if (precision != MSW_PREC_FP64)
-    return set_err(MSW_ERR_UNSUPPORTED, "precision %s%ld not built yet", "", precision);
+  if (precision != MSW_PREC_FP64 && precision != MSW_PREC_MIXED_FP32 &&
+      precision != MSW_PREC_MIXED_BF16)
+    return set_err(MSW_ERR_INVALID, "unknown precision %s%ld", "", precision);
+  if (precision != MSW_PREC_FP64 && opt->engine != MSW_ENGINE_MFMA)
+    return set_err(MSW_ERR_UNSUPPORTED,
+                   "mixed precision runs on the MFMA engine only (precision %s%ld)", "",
+                   precision);
   if (d->nm != 4 && d->nm != 8 && d->nm != 16)
     return set_err(MSW_ERR_UNSUPPORTED, "nm must be 4, 8 or 16 (got %s%ld)", "", d->nm);
   if (opt->engine != MSW_ENGINE_VALU && opt->engine != MSW_ENGINE_MFMA)
@@ -606,7 +611,7 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
   p->ntx = (d->ncell_x + tx - 1) / tx;
   p->nty = (d->ncell_y + ty - 1) / ty;
   if (p->engine == MSW_ENGINE_MFMA)
-    p->lds_bytes = mfma_f64_lds_bytes(p->nac, p->nwa);
+    p->lds_bytes = mfma_lds_bytes(precision, p->nac, p->nwa);
   else
     p->lds_bytes = sizeof(double) * ((size_t)p->nwa * 2 * kTile * (p->nac + 1) +
                                      (p->nwa > 1 ? (size_t)p->nwa * d->nm * 64 : 0));
@@ -622,15 +627,17 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
   const size_t nat = ((size_t)kNoct * p->na_pad + p->nac) * d->nm;
   const size_t nfyz = (size_t)kNoct * d->ne * d->nu * d->ncell_y * d->ncell_z * p->na_pad;
   const size_t nfxz = (size_t)kNoct * d->ne * d->nu * d->ncell_x * d->ncell_z * p->na_pad;
-  p->bytes = sizeof(double) * (2 * nat + nfyz + nfxz + 2 * kNormBlocks) +
+  const size_t fsz =
+      p->engine == MSW_ENGINE_MFMA ? mfma_face_elem_bytes(precision) : sizeof(double);
+  p->bytes = sizeof(double) * (2 * nat + 2 * kNormBlocks) + fsz * (nfyz + nfxz) +
              sizeof(TileItem) * flat.size();
   hipError_t e = hipSuccess;
   if (e == hipSuccess) e = hipMalloc(&p->at, sizeof(double) * nat);
   if (e == hipSuccess) e = hipMalloc(&p->mf, sizeof(double) * nat);
   if (e == hipSuccess) e = hipMemset(p->at, 0, sizeof(double) * nat);
   if (e == hipSuccess) e = hipMemset(p->mf, 0, sizeof(double) * nat);
-  if (e == hipSuccess) e = hipMalloc(&p->fyz, sizeof(double) * nfyz);
-  if (e == hipSuccess) e = hipMalloc(&p->fxz, sizeof(double) * nfxz);
+  if (e == hipSuccess) e = hipMalloc(&p->fyz, fsz * nfyz);
+  if (e == hipSuccess) e = hipMalloc(&p->fxz, fsz * nfxz);
   if (e == hipSuccess) e = hipMalloc(&p->norm_part, sizeof(double) * 2 * kNormBlocks);
   if (e == hipSuccess) e = hipMalloc(&p->items, sizeof(TileItem) * flat.size());
   if (e == hipSuccess)

@@ -1,0 +1,116 @@
+"""GPU mixed-precision sweep (FP32 or BF16 contractions on MFMA, FP32 face
+recurrence, FP64 state, vo accumulation and norms) against the FP64 oracle.
+
+Inputs: the deterministic manufactured state, and the same state perturbed
+elementwise by (1 + e), e ~ U(-2^-8, 2^-8) with a fixed seed.
+
+Stated error bars, relative to the FP64 result:
+  * manufactured state, both forms: vo and normsq exact.  Every operand
+    (state, matrices, faces) is a small integer times a power of two that
+    BF16/FP32 hold exactly, and all partial sums stay exact in FP32.
+  * perturbed state, FP32 form: vo normwise (max |d| / max |ref|) <= 1e-4 and
+    normsq <= 1e-4 (measured on MI355X: 1.1e-5 .. 2.8e-5 and 2.5e-6 .. 2.4e-5).
+    The floor is FP32 accumulation: the rows of A and M are second
+    differences weighted by 21..55 that cancel the smooth part of the input,
+    so rounding at 2^-24 of the summands shows up ~300x larger in the result.
+  * perturbed state, BF16 form: no accuracy claim.  Rounding the state and
+    the solved angular values to BF16 (2^-9) before the same cancellation
+    leaves O(1) normwise error (measured 0.5 .. 1.2); the test only checks
+    that the result is finite and within that measured band.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import minisweep_amd as msw
+from parity import closed_form_normsq
+
+pytestmark = pytest.mark.gpu
+
+PRECS = [msw.PREC_MIXED_FP32, msw.PREC_MIXED_BF16]
+PREC_IDS = ["fp32", "bf16"]
+# perturbed-input bars (the manufactured state must come out exact)
+VO_TOL = {msw.PREC_MIXED_FP32: 1e-4, msw.PREC_MIXED_BF16: 4.0}
+NORMSQ_TOL = {msw.PREC_MIXED_FP32: 1e-4, msw.PREC_MIXED_BF16: 8.0}
+
+SHAPES = [
+    # ncx, ncy, ncz, ne, nm, na, nu
+    (8, 8, 8, 4, 16, 32, 4),
+    (9, 7, 3, 2, 16, 64, 4),
+    (17, 3, 5, 1, 8, 40, 2),
+    (3, 20, 9, 3, 4, 13, 3),
+    (12, 10, 11, 2, 16, 64, 4),
+]
+
+
+def _sweep(d, vi0, prec, torch, device, nac=0):
+    a, m = msw.init_matrices(d.nm, d.na)
+    ta = torch.from_numpy(a).to(device)
+    tm = torch.from_numpy(m).to(device)
+    vi = torch.from_numpy(np.ascontiguousarray(vi0)).to(device)
+    vo = torch.full_like(vi, float("nan"))
+    plan = msw.Plan(d, prec, engine=msw.ENGINE_MFMA, angles_per_wave=nac)
+    plan.sweep(vi, vo, ta, tm)
+    torch.cuda.synchronize()
+    n, nd = msw.state_norms(d, vi, vo)
+    plan.close()
+    return vo.cpu().numpy(), n, nd
+
+
+def _errs(vo, ref):
+    normwise = float(np.max(np.abs(vo - ref)) / np.max(np.abs(ref)))
+    return normwise
+
+
+@pytest.mark.parametrize("prec", PRECS, ids=PREC_IDS)
+@pytest.mark.parametrize("shape", SHAPES, ids=[str(s) for s in SHAPES])
+def test_mixed_manufactured(shape, prec, cuda):
+    import torch
+    ncx, ncy, ncz, ne, nm, na, nu = shape
+    print(f"case {shape} prec {prec}", flush=True)
+    d = msw.dims(*shape)
+    vi0 = oracle.init_state(oracle.dims(*shape))
+    vo, n, nd = _sweep(d, vi0, prec, torch, cuda)
+    assert np.isfinite(vo).all()
+    err = _errs(vo, vi0)
+    exact = float(closed_form_normsq(ncx, ncy, ncz, ne, nm, nu))
+    rel_n = abs(n - exact) / exact
+    print(f"  vo normwise err {err:.3e}  normsq rel err {rel_n:.3e}", flush=True)
+    np.testing.assert_array_equal(vo, vi0)
+    assert n == exact and nd == 0.0
+
+
+@pytest.mark.parametrize("prec", PRECS, ids=PREC_IDS)
+@pytest.mark.parametrize("shape", SHAPES, ids=[str(s) for s in SHAPES])
+def test_mixed_perturbed_vs_oracle(shape, prec, cuda):
+    import torch
+    print(f"case {shape} prec {prec}", flush=True)
+    d = msw.dims(*shape)
+    od = oracle.dims(*shape)
+    rng = np.random.default_rng(1234)
+    base = oracle.init_state(od)
+    vi0 = base * (1.0 + rng.uniform(-2.0 ** -8, 2.0 ** -8, base.size))
+    vo, n, nd = _sweep(d, vi0, prec, torch, cuda)
+    a, m = oracle.init_matrices(od)
+    ref = oracle.sweep(od, a, m, vi0)
+    n_ref, _ = oracle.norms(od, vi0, ref)
+    err = _errs(vo, ref)
+    rel_n = abs(n - n_ref) / n_ref
+    print(f"  vo normwise err {err:.3e}  normsq rel err {rel_n:.3e}", flush=True)
+    assert np.isfinite(vo).all()
+    assert err <= VO_TOL[prec]
+    assert rel_n <= NORMSQ_TOL[prec]
+
+
+@pytest.mark.parametrize("prec", PRECS, ids=PREC_IDS)
+def test_mixed_deterministic_and_chunking(prec, cuda):
+    """Same bits on a rerun; angle chunking changes only the summation order."""
+    import torch
+    shape = (9, 7, 5, 2, 16, 64, 4)
+    d = msw.dims(*shape)
+    vi0 = oracle.init_state(oracle.dims(*shape))
+    r1, _, _ = _sweep(d, vi0, prec, torch, cuda, nac=16)
+    r2, _, _ = _sweep(d, vi0, prec, torch, cuda, nac=16)
+    np.testing.assert_array_equal(r1, r2)
+    r3, _, _ = _sweep(d, vi0, prec, torch, cuda, nac=64)
+    np.testing.assert_array_equal(r3, vi0)  # exact either way on this input
