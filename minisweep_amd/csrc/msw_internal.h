@@ -23,6 +23,9 @@ struct LaunchSpec {
 // (MSW_PREC_*).  nac is 16, 32 or 64; nwa 1..4; nm 4, 8 or 16.
 void launch_mfma(int precision, const LaunchSpec& ls, const SweepParams& sp, hipStream_t st);
 size_t mfma_lds_bytes(int precision, int nac, int nwa);
+// Writes the domain-boundary inflow into the MFMA kernels' face planes; runs
+// before the launches of every sweep.
+void launch_mfma_fill(int precision, const SweepParams& sp, hipStream_t st);
 // Bytes of one face value of the MFMA kernels in a precision.
 size_t mfma_face_elem_bytes(int precision);
 // Tile of one wave of the MFMA kernels (cells along x and y).

@@ -33,7 +33,9 @@
 //   * Face planes in HBM hold, per (octant, group, unknown), one yz plane for
 //     x faces and one xz plane for y faces, angles in lane order.  The host
 //     schedule never runs two tiles of one octant row (column) in one launch,
-//     so a tile reads its inflow and writes its outflow in place.
+//     so a tile reads its inflow and writes its outflow in place.  A pre-pass
+//     fills the planes with the domain-boundary inflow, so every tile takes
+//     its inflow the same way; only the z boundary is computed in the kernel.
 //
 // MFMA layouts (gfx950):
 //   f64 16x16x4 : A[i][k] lane i+16k, B[k][n] lane n+16k, D[i][n] lane n+16(i%4) reg i/4
@@ -45,6 +47,8 @@
 // Quantities_solve (src/3_sweeper/quantities_testing_kernels.h:380-446), the
 // boundary inflow Quantities_init_face (same file, :219-332).
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "msw_device.h"
 #include "msw_internal.h"
@@ -112,6 +116,7 @@ struct PrecF64 {
   typedef double A;
   typedef d4v acc4;
   static constexpr bool kPacked = false;
+  static constexpr int kWz = 2;  // (g + 4r) & 7 depends on r & 1 only
   static __device__ __forceinline__ constexpr int row(int g, int r) { return g + 4 * r; }
   static __device__ __forceinline__ constexpr int mom(int i) { return 4 * (i & 3) + (i >> 2); }
   static __device__ __forceinline__ acc4 mma(T a, T b, acc4 c) {
@@ -123,6 +128,7 @@ struct PrecF32 {
   typedef float A;
   typedef f4v acc4;
   static constexpr bool kPacked = false;
+  static constexpr int kWz = 4;
   static __device__ __forceinline__ constexpr int row(int g, int r) { return 4 * g + r; }
   static __device__ __forceinline__ constexpr int mom(int i) { return i; }
   static __device__ __forceinline__ acc4 mma(T a, T b, acc4 c) {
@@ -134,6 +140,7 @@ struct PrecBF16 {
   typedef float A;
   typedef f4v acc4;
   static constexpr bool kPacked = true;  // K = 16: 4 values per lane per MFMA
+  static constexpr int kWz = 4;
   static __device__ __forceinline__ constexpr int row(int g, int r) { return 4 * g + r; }
   static __device__ __forceinline__ constexpr int mom(int i) { return i; }
   static __device__ __forceinline__ acc4 mma16(const T (&a)[4], const T (&b)[4], acc4 c) {
@@ -157,8 +164,17 @@ __host__ __device__ __forceinline__ unsigned coef_lut(int shift) {
   return l;
 }
 
+// Waves per SIMD the register allocation aims for: two where the working set
+// fits in 256 registers (FP64 with 16 angles per wave, FP32 with up to 32),
+// so that one wave's matrix work overlaps the other's solve and loads.
+template <class PR, int NAC>
+struct Occupancy {
+  static constexpr int value = sizeof(typename PR::T) * NAC <= 128 ? 2 : 1;
+};
+
 template <class PR, int NM, int NAC, int NWA>
-__global__ __launch_bounds__(64 * NWA) void k_sweep_mfma(SweepParams p, int item_base) {
+__global__ __launch_bounds__(64 * NWA) __attribute__((amdgpu_waves_per_eu(
+    Occupancy<PR, NAC>::value))) void k_sweep_mfma(SweepParams p, int item_base) {
   typedef typename PR::T T;
   typedef typename PR::A A;
   typedef typename PR::acc4 acc4;
@@ -185,33 +201,33 @@ __global__ __launch_bounds__(64 * NWA) void k_sweep_mfma(SweepParams p, int item
   const int nx = p.nx, ny = p.ny, nz = p.nz, nu = p.nu, nap = p.na_pad;
   const int gx = dx > 0 ? it.tx : p.ntx - 1 - it.tx;
   const int gy = dy > 0 ? it.ty : p.nty - 1 - it.ty;
-  // A partial last tile swept downward starts at the first lane inside.
-  const int xoff = (dx < 0 && it.tx == 0) ? p.ntx * kTX - nx : 0;
-  const int yoff = (dy < 0 && it.ty == 0) ? p.nty * kTY - ny : 0;
   const int ia0 = wave * NAC;
   const bool first = it.first != 0;
-  const bool x_bnd = it.tx == 0, y_bnd = it.ty == 0;
   const bool x_out = it.tx + 1 < p.ntx, y_out = it.ty + 1 < p.nty;
 
-  // ---- the lane's cells: (x, y_b) for b = 0..3 ----------------------------
-  const int x = dx > 0 ? gx * kTX + j : gx * kTX + kTX - 1 - j;
-  const bool x_ok = x < nx;
-  const int xc = x_ok ? x : nx - 1;  // clamped: loads never leave the arrays
+  // ---- the lane's cells ------------------------------------------------------
+  // The tile covers grid cells [16 gx, 16 gx + wx) x [4 gy, 4 gy + wy) for
+  // every octant; lane j = 0 and block b = 0 are its upwind-most column and
+  // row in the octant's direction, so a partial tile's missing cells are
+  // always downstream (lanes j >= wx, blocks b >= wy).
+  const int wx = min(kTX, nx - gx * kTX), wy = min(kTY, ny - gy * kTY);
+  const bool x_ok = j < wx;
+  const int jc = x_ok ? j : wx - 1;  // clamped: loads never leave the arrays
+  const int x = dx > 0 ? gx * kTX + jc : gx * kTX + wx - 1 - jc;
   int yb[kTY], cst[kTY];
-  bool y_ok[kTY];
   const size_t Se = (size_t)ny * nx * nu * NM;  // state stride of a group
   const size_t Sz = (size_t)p.ne * Se;          // state stride of a z plane
-  size_t coff[kTY];                             // state offset of the cell at z = 0
+  // state offset of the lane's cells at z = 0: a per-lane x part plus a
+  // per-block y part that is the same for all lanes
+  const size_t cxo = (size_t)ie * Se + ((size_t)x * nu + iu) * NM;
+  size_t coff[kTY];
 #pragma unroll
   for (int b = 0; b < kTY; ++b) {
-    yb[b] = dy > 0 ? gy * kTY + b : gy * kTY + kTY - 1 - b;
-    y_ok[b] = yb[b] < ny;
-    const int yc = y_ok[b] ? yb[b] : ny - 1;
-    coff[b] = (size_t)ie * Se + (((size_t)yc * nx + xc) * nu + iu) * NM;
+    const int bc = b < wy ? b : wy - 1;
+    yb[b] = dy > 0 ? gy * kTY + bc : gy * kTY + wy - 1 - bc;
+    coff[b] = cxo + (size_t)yb[b] * nx * nu * NM;
     cst[b] = x + 3 * yb[b] + 2;
   }
-  const int xb = dx > 0 ? -1 : nx;
-  const int ybnd = dy > 0 ? -1 : ny;
   const int zb = dz > 0 ? -1 : nz;
 
   // ---- LDS: vo partial sums [2][kTY][NWA][64][4], x inflow staging [NWA][NAB][64]
@@ -227,10 +243,10 @@ __global__ __launch_bounds__(64 * NWA) void k_sweep_mfma(SweepParams p, int item
 
   // ---- per-octant operands, in registers for the whole launch --------------
   const int moff = MSTEP * g;
-  T aop[NAB][KA];  // m2a A: AT[ia0 + 16a + j][moff + s]
-  T mop[NAB][4];   // a2m A: M[ia0 + 16a + row(g, s)][mom(j)]
-  T wz[4];         // zfluxweight of the lane's angles: 1/4 - 2^-(ia & 7)
-  T iaf[NAB][4];   // (1 + ia) 2^(ia & 7): angle factor of the boundary inflow
+  T aop[NAB][KA];      // m2a A: AT[ia0 + 16a + j][moff + s]
+  T mop[NAB][4];       // a2m A: M[ia0 + 16a + row(g, s)][mom(j)]
+  T wz[PR::kWz];       // zfluxweight of the lane's angles: 1/4 - 2^-(ia & 7), by r % kWz
+  float iaf[NAB][4];   // (1 + ia) 2^(ia & 7) (exact in FP32): boundary inflow angle factor
   {
     const double* at = p.at + ((size_t)oct * nap + ia0) * NM;
     const double* mf = p.mf + ((size_t)oct * nap + ia0) * NM;
@@ -246,11 +262,11 @@ __global__ __launch_bounds__(64 * NWA) void k_sweep_mfma(SweepParams p, int item
       for (int s = 0; s < 4; ++s) {
         mop[a][s] = mm < NM ? (T)mf[(16 * a + PR::row(g, s)) * NM + (mm < NM ? mm : 0)] : (T)0;
         const int ia = ia0 + 16 * a + PR::row(g, s);
-        iaf[a][s] = ia < p.na ? (T)((1 + ia) << (ia & 7)) : (T)0;
+        iaf[a][s] = ia < p.na ? (float)((1 + ia) << (ia & 7)) : 0.0f;
       }
     }
 #pragma unroll
-    for (int s = 0; s < 4; ++s) wz[s] = (T)0.25 - ldexpT((T)1, -(PR::row(g, s) & 7));
+    for (int s = 0; s < PR::kWz; ++s) wz[s] = (T)0.25 - ldexpT((T)1, -(PR::row(g, s) & 7));
   }
   const T eu = (T)(sf_energy(ie) * sf_unknown(iu));
   const unsigned lut_z = coef_lut(dz), lut_y = coef_lut(dy), lut_x = coef_lut(-dx);
@@ -264,7 +280,9 @@ __global__ __launch_bounds__(64 * NWA) void k_sweep_mfma(SweepParams p, int item
       for (int r = 0; r < 4; ++r) F[a][b][r] = (T)0;
 
   // ---- data fetched one step ahead; each item is refetched for the next step
-  // right after its last use in this one -------------------------------------
+  // right after its last use in this one.  Every prefetch is issued on every
+  // path, the last step's too (z is clamped), so that the memory counters
+  // agree across paths and each wait covers only the data it needs. ---------
   T vin[kTY][KA];        // m2a B operand
   double von[NOWN][4];   // old vo of the owned blocks' cells
   T xsn[NAB];            // x inflow, spread: lane (j, g) holds block j/4, register j%4
@@ -299,34 +317,31 @@ __global__ __launch_bounds__(64 * NWA) void k_sweep_mfma(SweepParams p, int item
   };
   auto fetch_xs = [&](int t) {
     const int bb = j >> 2;
-    int y = dy > 0 ? gy * kTY + bb : gy * kTY + kTY - 1 - bb;
-    y = y < ny ? y : ny - 1;
+    const int bc = bb < wy ? bb : wy - 1;
+    const int y = dy > 0 ? gy * kTY + bc : gy * kTY + wy - 1 - bc;
     const T* src = fyz + ((size_t)y * nz + zclamp(t - bb)) * nap + (j & 3);
 #pragma unroll
     for (int a = 0; a < NAB; ++a) xsn[a] = src[16 * a];
   };
   auto fetch_yin = [&](int t) {
-    const T* src = fxz + ((size_t)xc * nz + zclamp(t - j)) * nap;
+    const T* src = fxz + ((size_t)x * nz + zclamp(t - j)) * nap;
 #pragma unroll
     for (int a = 0; a < NAB; ++a) load4(src + 16 * a, yinn[a]);
   };
 
+  fetch_xs(0);
   fetch_vi(0);
-  if (!first) fetch_vo(0);
-  if (!x_bnd) fetch_xs(0);
-  if (!y_bnd) fetch_yin(0);
+  fetch_yin(0);
+  fetch_vo(0);
 
   for (int t = 0; t < nsteps; ++t) {
-    const bool more = t + 1 < nsteps;
     // x inflow of this step into LDS: lane (0, g) reads its 16 values back
-    if (!x_bnd) {
 #pragma unroll
-      for (int a = 0; a < NAB; ++a) xs_lds[a * 64 + lane] = xsn[a];
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      if (more) fetch_xs(t + 1);
-    }
+    for (int a = 0; a < NAB; ++a) xs_lds[a * 64 + lane] = xsn[a];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    fetch_xs(t + 1);
 
     // ---- per cell block: validity, z, spatial phase; z inflow on entry -------
     bool val[kTY];
@@ -334,7 +349,7 @@ __global__ __launch_bounds__(64 * NWA) void k_sweep_mfma(SweepParams p, int item
 #pragma unroll
     for (int b = 0; b < kTY; ++b) {
       const int zp = t - j - b;
-      val[b] = x_ok && y_ok[b] && (unsigned)zp < (unsigned)nz;
+      val[b] = x_ok && b < wy && (unsigned)zp < (unsigned)nz;
       zgs[b] = zclamp(zp);
       sh[b] = ((cst[b] + 7 * zgs[b]) & 3) * 4;
       if (zp == 0) {
@@ -342,16 +357,8 @@ __global__ __launch_bounds__(64 * NWA) void k_sweep_mfma(SweepParams p, int item
 #pragma unroll
         for (int a = 0; a < NAB; ++a)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) F[a][b][r] = iaf[a][r] * sc;
+          for (int r = 0; r < 4; ++r) F[a][b][r] = (T)iaf[a][r] * sc;
       }
-    }
-    // y inflow of block 0 on the domain boundary
-    if (y_bnd) {
-      const T sc = ldexpT(eu, sf_space_exp(x, ybnd, zclamp(t - j - yoff)));
-#pragma unroll
-      for (int a = 0; a < NAB; ++a)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) yinn[a][r] = iaf[a][r] * sc;
     }
 
     acc4 vacc[kTY];
@@ -371,47 +378,25 @@ __global__ __launch_bounds__(64 * NWA) void k_sweep_mfma(SweepParams p, int item
           for (int s = 0; s < KM; ++s) vs[b] = PR::mma(aop[a][s], vin[b][s], vs[b]);
         }
       }
-      if (a == NAB - 1 && more) fetch_vi(t + 1);
+      if (a == NAB - 1) fetch_vi(t + 1);
       // solve, blocks in descending order: block b's y face is block b-1's
       // value from the previous step, read before it is overwritten
 #pragma unroll
       for (int b = kTY - 1; b >= 0; --b) {
         T xin[4];
-        if (x_bnd) {
-          const T sc = ldexpT(eu, sf_space_exp(xb, yb[b], zgs[b]));
-#pragma unroll
-          for (int r = 0; r < 4; ++r) xin[r] = iaf[a][r] * sc;
-        } else {
-          load4(xs_lds + a * 64 + 16 * g + 4 * b, xin);
-        }
+        load4(xs_lds + a * 64 + 16 * g + 4 * b, xin);
         const T cz = ldexpT((T)0x1p-9, (int)((lut_z >> sh[b]) & 15));
         const T cy = ldexpT((T)0x1p-10, (int)((lut_y >> sh[b]) & 15));
         const int xe = (int)((lut_x >> sh[b]) & 15) - 8;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const T rx = shr1_keep(xin[r], F[a][b][r]);
-          const T ry = b == 0 ? yinn[a][r] : F[a][b > 0 ? b - 1 : 0][r];
-          const T res = fmaT(rx, ldexpT(wz[r], xe),
+          const T ry = b > 0 ? F[a][b > 0 ? b - 1 : 0][r] : yinn[a][r];
+          const T res = fmaT(rx, ldexpT(wz[r % PR::kWz], xe),
                              fmaT(ry, cy, fmaT(F[a][b][r], cz, (T)vs[b][r])));
           F[a][b][r] = res;
           vs[b][r] = res;
         }
-        // Lanes left of a partial tile's first cell carry the boundary inflow
-        // so that the first cell reads it from its left neighbour next step.
-        if (x_bnd && xoff != 0 && !x_ok) {
-#pragma unroll
-          for (int r = 0; r < 4; ++r) F[a][b][r] = xin[r];
-        }
-      }
-      // Blocks below a partial tile's first row carry the y boundary inflow.
-      if (y_bnd && yoff != 0) {
-#pragma unroll
-        for (int b = 0; b < kTY - 1; ++b)
-          if (b < yoff) {
-            const T sc = ldexpT(eu, sf_space_exp(x, ybnd, zgs[b]));
-#pragma unroll
-            for (int r = 0; r < 4; ++r) F[a][b][r] = iaf[a][r] * sc;
-          }
       }
       // a2m: VO[moment][cell] += M^T[moment][angle] R[angle][cell]
 #pragma unroll
@@ -425,10 +410,10 @@ __global__ __launch_bounds__(64 * NWA) void k_sweep_mfma(SweepParams p, int item
         }
       }
     }
-    if (!y_bnd && more) fetch_yin(t + 1);
+    fetch_yin(t + 1);
 
     // ---- outflow faces for the downstream tiles ------------------------------
-    if (x_out && j == kTX - 1) {
+    if (x_out && j == wx - 1) {
 #pragma unroll
       for (int b = 0; b < kTY; ++b)
         if (val[b]) {
@@ -437,10 +422,14 @@ __global__ __launch_bounds__(64 * NWA) void k_sweep_mfma(SweepParams p, int item
           for (int a = 0; a < NAB; ++a) store4(dst + 16 * a, F[a][b]);
         }
     }
-    if (y_out && val[kTY - 1]) {
-      T* dst = fxz + ((size_t)x * nz + zgs[kTY - 1]) * nap;
+    if (y_out) {
 #pragma unroll
-      for (int a = 0; a < NAB; ++a) store4(dst + 16 * a, F[a][kTY - 1]);
+      for (int b = 0; b < kTY; ++b)
+        if (b == wy - 1 && val[b]) {
+          T* dst = fxz + ((size_t)x * nz + zgs[b]) * nap;
+#pragma unroll
+          for (int a = 0; a < NAB; ++a) store4(dst + 16 * a, F[a][b]);
+        }
     }
 
     // ---- vo: lane (j, g) holds moments 4g..4g+3 of cell (j, b) ---------------
@@ -477,7 +466,47 @@ __global__ __launch_bounds__(64 * NWA) void k_sweep_mfma(SweepParams p, int item
       }
       if (val[b] && 4 * g < NM) store4(p.vo + coff[b] + (size_t)zgs[b] * Sz + 4 * g, sum);
     }
-    if (!first && more) fetch_vo(t + 1);
+    fetch_vo(t + 1);
+  }
+}
+
+// Boundary inflow of every tile row and column: before the sweep, the x face
+// plane of each (octant, group, unknown) holds the x-boundary value of every
+// (y, z, angle) and the y face plane the y-boundary value of every (x, z,
+// angle); the first tile of a row (column) reads them like any other inflow.
+// Values are Quantities_init_face's divided by (1 + octant), angles in the
+// lane order of the sweep kernel.
+template <class PR>
+__global__ void k_fill_faces(SweepParams p) {
+  typedef typename PR::T T;
+  const int nap = p.na_pad, nz = p.nz;
+  const size_t per_ouy_x = (size_t)p.ny * nz * nap;
+  const size_t per_ouy_y = (size_t)p.nx * nz * nap;
+  const size_t n_x = (size_t)kNoct * p.ne * p.nu * per_ouy_x;
+  const size_t n = n_x + (size_t)kNoct * p.ne * p.nu * per_ouy_y;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const bool xface = i < n_x;
+    size_t k = xface ? i : i - n_x;
+    const int pos = (int)(k % nap);
+    k /= nap;
+    const int z = (int)(k % nz);
+    k /= nz;
+    const int ext = xface ? p.ny : p.nx;
+    const int c = (int)(k % ext);  // y for x faces, x for y faces
+    k /= ext;
+    const int iu = (int)(k % p.nu);
+    k /= p.nu;
+    const int ie = (int)(k % p.ne);
+    const int oct = (int)(k / p.ne);
+    const int ia = 16 * (pos >> 4) + PR::row((pos & 15) >> 2, pos & 3);
+    const int e = xface ? sf_space_exp((oct & 1) ? p.nx : -1, c, z)
+                        : sf_space_exp(c, (oct & 2) ? p.ny : -1, z);
+    const double v = ia < p.na ? ldexp((double)((1 + ia) << (ia & 7)) * sf_energy(ie) *
+                                           sf_unknown(iu), e)
+                               : 0.0;
+    (xface ? reinterpret_cast<T*>(p.fyz) : reinterpret_cast<T*>(p.fxz))[i - (xface ? 0 : n_x)] =
+        (T)v;
   }
 }
 
@@ -530,6 +559,15 @@ void launch_mfma(int precision, const LaunchSpec& ls, const SweepParams& sp, hip
     launch_prec<PrecBF16>(ls, sp, st);
   else
     launch_prec<PrecF64>(ls, sp, st);
+}
+
+void launch_mfma_fill(int precision, const SweepParams& sp, hipStream_t st) {
+  const size_t n = (size_t)kNoct * sp.ne * sp.nu * (sp.nx + sp.ny) * sp.nz * sp.na_pad;
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+  if (precision == MSW_PREC_FP64)
+    hipLaunchKernelGGL(k_fill_faces<PrecF64>, dim3(blocks), dim3(256), 0, st, sp);
+  else
+    hipLaunchKernelGGL(k_fill_faces<PrecF32>, dim3(blocks), dim3(256), 0, st, sp);
 }
 
 size_t mfma_face_elem_bytes(int precision) {

@@ -580,11 +580,12 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
   p->precision = precision;
   p->engine = opt->engine;
   if (p->engine == MSW_ENGINE_MFMA) {
-    // 16-angle MFMA blocks: 16 or 32 angles per wave
-    // 16 angles per wave keeps the pipelined kernel in registers; larger
-    // angle counts move to 32 or 64 so that at most 4 waves share a cell tile
-    p->nac = opt->angles_per_wave ? opt->angles_per_wave
-                                  : (d->na <= 64 ? 16 : d->na <= 128 ? 32 : 64);
+    // 16-angle MFMA blocks: 16, 32 or 64 angles per wave, at most 4 waves
+    // (angle chunks) per cell tile.  Measured at 64^3, na=64: 32 angles per wave is fastest in FP64 (16
+    // loses more to the cross-wave vo reduction than it gains in occupancy),
+    // 64 in the mixed forms
+    const int def = precision == MSW_PREC_FP64 ? (d->na <= 128 ? 32 : 64) : 64;
+    p->nac = opt->angles_per_wave ? opt->angles_per_wave : def;
     if (p->nac != 16 && p->nac != 32 && p->nac != 64) {
       delete p;
       return set_err(MSW_ERR_INVALID,
@@ -732,6 +733,7 @@ int msw_plan_sweep(MswPlan* p, const double* vi, double* vo,
   sp.na_pad = p->na_pad;
   sp.ntx = p->ntx;
   sp.nty = p->nty;
+  if (p->engine == MSW_ENGINE_MFMA) launch_mfma_fill(p->precision, sp, st);
   switch (d.nm) {
     case 4: return launch_nm<4>(p, sp, st);
     case 8: return launch_nm<8>(p, sp, st);
