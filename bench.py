@@ -1,21 +1,24 @@
 #!/usr/bin/env python3
 """Sweep throughput on one MI355X.
 
-Workload (default): the FP64 plain sweep at one GPU's domain -- 64^3 cells,
-ne=64 energy groups, na=64 angles per octant, nm=16 moments, nu=4 unknowns,
-8 octants -- with vi/vo (8.6 GB each) resident in HBM and the deterministic
+Workload: the plain FP64 sweep at one GPU's domain -- 64^3 cells, ne=64
+energy groups, na=64 angles per octant, nm=16 moments, nu=4 unknowns, 8
+octants -- with vi/vo (8.6 GB each) resident in HBM and the deterministic
 Quantities_init_* input state.  One step = one sweep vo <- S(vi); steps
 ping-pong vi/vo as the reference runner does (src/4_driver/runner.c:148-155).
 GFLOP/s uses the reference's flop count (src/4_driver/runner.c:162-166).
 
-Prints ONE JSON line.  Also reported in it:
-  roofline     dominant kernel (k_sweep_fp64) vs the FP64 peak, from per-launch
-               HIP events recorded on the sweep's stream during the timed
-               steps (the last timed sweep's launches).
+Prints ONE JSON line.  `value` is the FP64 sweep's GFLOP/s.  Also in it:
+  mixed        the FP32 and BF16 mixed-precision forms on the same workload:
+               GFLOP/s, speedup over FP64, normsq error on the manufactured
+               input, and the per-output error against the FP64 sweep on a
+               perturbed input vi*(1+e), e ~ U(-2^-8, 2^-8) (seed 0).
+  roofline     the dominant kernel (k_sweep_mfma) against the FP64 peak, from
+               per-launch HIP events recorded on the sweep's stream during the
+               timed steps; HBM traffic from the committed rocprofv3 summary.
   cpu_baseline the CPU oracle (oracle/, a restatement of the reference
-               algorithm) timed on this host over a bounded sample: the same
-               64^3 x na64 x nm16 x nu4 shape with a few energy groups, one
-               sweep, OpenMP over energy groups.
+               algorithm) timed on this host over a bounded sample of the same
+               shape: a few energy groups, one sweep, OpenMP over groups.
 """
 from __future__ import annotations
 
@@ -32,6 +35,8 @@ METRIC = ("sweep GFLOP/s + p50/p99 latency, mixed vs FP64 plain at ≤1e-5 rel e
           "1 GPU")
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+TRAFFIC_FILES = {"fp64": "k_sweep_mfma_f64_64cube_na64.md",
+                 "fp32": "k_sweep_mfma_mixed_fp32_64cube_na64.md"}
 
 
 def closed_form_normsq(ncx, ncy, ncz, ne, nm, nu):
@@ -44,19 +49,21 @@ def closed_form_normsq(ncx, ncy, ncz, ne, nm, nu):
 
 
 def read_traffic(name):
-    """HBM bytes per sweep from the committed rocprofv3 PMC summary, if any."""
+    """(bytes per sweep, bytes per launch) from a committed rocprofv3 summary."""
     path = os.path.join(REPO, "profiles", name)
     if not os.path.exists(path):
-        return None
+        return None, None
+    per_sweep = per_launch = None
     for line in open(path):
         if line.startswith("hbm_bytes_per_sweep:"):
-            return float(line.split(":", 1)[1])
-    return None
+            per_sweep = float(line.split(":", 1)[1])
+        if line.startswith("hbm_bytes_per_launch:"):
+            per_launch = float(line.split(":", 1)[1])
+    return per_sweep, per_launch
 
 
 def cpu_baseline(ncx, ncy, ncz, nm, na, nu, threads):
     """Oracle on this host: a few energy groups of the same shape, one sweep."""
-    import numpy as np  # noqa: F401
     import oracle
     ne_s = max(1, threads)
     d = oracle.dims(ncx, ncy, ncz, ne_s, nm, na, nu)
@@ -77,6 +84,48 @@ def cpu_baseline(ncx, ncy, ncz, nm, na, nu, threads):
     }
 
 
+def pct(vals, p):
+    s = sorted(vals)
+    k = min(len(s) - 1, max(0, int(round(p / 100 * (len(s) - 1)))))
+    return s[k]
+
+
+def time_precision(msw, torch, d, prec_name, args, ta, tm, vi, vo, flops_sweep):
+    """Warmup + exactly `steps` timed sweeps of one precision form."""
+    prec = {"fp64": msw.PREC_FP64, "fp32": msw.PREC_MIXED_FP32,
+            "bf16": msw.PREC_MIXED_BF16}[prec_name]
+    engine = msw.ENGINE_MFMA if args.engine == "mfma" else msw.ENGINE_VALU
+    plan = msw.Plan(d, prec, engine=engine, angles_per_wave=args.angles_per_wave)
+    stream = torch.cuda.current_stream()
+    bufs = [vi, vo]
+    for i in range(args.warmup):
+        plan.sweep(bufs[i % 2], bufs[(i + 1) % 2], ta, tm, stream)
+    msw.init_state(d, vi, stream)  # the timed steps start from the same input
+    plan.set_timing(True)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for i in range(args.steps):
+        plan.sweep(bufs[i % 2], bufs[(i + 1) % 2], ta, tm, stream)
+        ev[i + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]
+    launch_ms = plan.last_timings_ms()
+    plan.set_timing(False)
+    last_out, last_in = bufs[args.steps % 2], bufs[(args.steps + 1) % 2]
+    normsq, normsqdiff = msw.state_norms(d, last_in, last_out)
+    return plan, {
+        "value": flops_sweep * args.steps / wall / 1e9,
+        "ms_per_step": wall / args.steps * 1e3,
+        "lat": lat,
+        "launch_ms": launch_ms,
+        "normsq": normsq,
+        "normsqdiff": normsqdiff,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,7 +137,9 @@ def main():
     ap.add_argument("--nm", type=int, default=16)
     ap.add_argument("--nu", type=int, default=4)
     ap.add_argument("--engine", choices=["mfma", "valu"], default="mfma")
-    ap.add_argument("--precision", choices=["fp64", "fp32", "bf16"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "fp32", "bf16"], default=None,
+                    help="time only this form (default: FP64 headline plus both "
+                         "mixed forms)")
     ap.add_argument("--angles-per-wave", type=int, default=0)
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="threads for the CPU baseline (0: OMP_NUM_THREADS or 16)")
@@ -114,77 +165,79 @@ def main():
     d = msw.dims(n, n, n, args.ne, args.nm, args.na, args.nu)
     size = msw.size_state(d)
     flops_sweep = msw.flops(d, 1)
+    expect = float(closed_form_normsq(n, n, n, args.ne, args.nm, args.nu))
+    default_shape = (n, args.ne, args.na, args.nm, args.nu) == (64, 64, 64, 16, 4)
 
     a, m = msw.init_matrices(d.nm, d.na)
     ta, tm = torch.from_numpy(a).to(dev), torch.from_numpy(m).to(dev)
     vi = torch.empty(size, dtype=torch.float64, device=dev)
     vo = torch.empty(size, dtype=torch.float64, device=dev)
     msw.init_state(d, vi)
-    engine = msw.ENGINE_MFMA if args.engine == "mfma" else msw.ENGINE_VALU
-    prec = {"fp64": msw.PREC_FP64, "fp32": msw.PREC_MIXED_FP32,
-            "bf16": msw.PREC_MIXED_BF16}[args.precision]
-    plan = msw.Plan(d, prec, engine=engine, angles_per_wave=args.angles_per_wave)
-    stream = torch.cuda.current_stream()
 
-    bufs = [vi, vo]
-    for i in range(args.warmup):
-        plan.sweep(bufs[i % 2], bufs[(i + 1) % 2], ta, tm)
-    # restart from the initial state so the timed steps see the same input
-    msw.init_state(d, vi)
-    torch.cuda.synchronize()
+    forms = [args.precision] if args.precision else ["fp64", "fp32", "bf16"]
+    head = forms[0]
+    res, plans = {}, {}
+    for f in forms:
+        plans[f], res[f] = time_precision(msw, torch, d, f, args, ta, tm, vi, vo, flops_sweep)
 
-    plan.set_timing(True)
-    lat = []
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev[0].record(stream)
-    for i in range(args.steps):
-        plan.sweep(bufs[i % 2], bufs[(i + 1) % 2], ta, tm, stream)
-        ev[i + 1].record(stream)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for i in range(args.steps):
-        lat.append(ev[i].elapsed_time(ev[i + 1]))
-    launch_ms = plan.last_timings_ms()
-    plan.set_timing(False)
+    # per-output error of the mixed forms against the FP64 sweep, full size
+    errs = {}
+    if len(forms) > 1:
+        msw.init_state(d, vi)
+        g = torch.Generator(device=dev)
+        g.manual_seed(0)
+        vi.mul_(1.0 + (torch.rand(size, dtype=torch.float64, device=dev, generator=g)
+                       * 2.0 - 1.0) * 2.0 ** -8)
+        ref = torch.empty_like(vi)
+        plans["fp64"].sweep(vi, ref, ta, tm)
+        n_ref, _ = msw.state_norms(d, vi, ref)
+        ref_max = float(ref.abs().max())
+        for f in forms[1:]:
+            plans[f].sweep(vi, vo, ta, tm)
+            n_f, _ = msw.state_norms(d, vi, vo)
+            vo.sub_(ref)
+            errs[f] = {"vo_normwise": float(vo.abs().max()) / ref_max,
+                       "normsq_rel": abs(n_f - n_ref) / n_ref}
+        torch.cuda.synchronize()
 
-    wall = t1 - t0
-    value = flops_sweep * args.steps / wall / 1e9
+    r = res[head]
+    launch_ms = r["launch_ms"]
     kernel_ms = sum(launch_ms)
-
-    # correctness of the timed run: the manufactured problem maps vi to vi
-    last_out = bufs[args.steps % 2]
-    last_in = bufs[(args.steps + 1) % 2]
-    normsq, normsqdiff = msw.state_norms(d, last_in, last_out)
-    expect = float(closed_form_normsq(n, n, n, args.ne, args.nm, args.nu))
-    exact = normsqdiff == 0.0 and normsq == expect
-    normsq_rel_err = abs(normsq - expect) / expect
-
-    lat_sorted = sorted(lat)
-
-    def pct(p):
-        k = min(len(lat_sorted) - 1, max(0, int(round(p / 100 * (len(lat_sorted) - 1)))))
-        return lat_sorted[k]
-
     achieved_tf = flops_sweep / (kernel_ms * 1e-3) / 1e12
-    traffic = read_traffic("k_sweep_fp64_64cube_traffic.txt")
+    t_sweep, t_launch = (read_traffic(TRAFFIC_FILES[head])
+                         if default_shape and head in TRAFFIC_FILES else (None, None))
     roofline = {
         "bound": "mfma",
-        "pipe": ("FP64 MFMA 16x16x4 for m2a/a2m, VALU for the solve" if args.engine == "mfma"
-                 else "FP64 vector FMA")
-                + " (MI355X FP64 vector and matrix peaks are both 78.6 TF)",
-        "kernel": "k_sweep_mfma_f64" if args.engine == "mfma" else "k_sweep_fp64",
+        "kernel": f"k_sweep_mfma ({head})" if args.engine == "mfma" else "k_sweep_fp64",
         "achieved": round(achieved_tf, 3),
         "peak": PEAK_FP64_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved_tf / PEAK_FP64_TFLOPS, 4),
-        "traffic": traffic,
+        "traffic": t_launch,
+        "traffic_per_sweep": t_sweep,
+        "traffic_GBs": (round(t_sweep / (kernel_ms * 1e-3) / 1e9, 1) if t_sweep else None),
         "launches_per_sweep": len(launch_ms),
         "kernel_ms_per_sweep": round(kernel_ms, 3),
+        "kernel_avg_launch_ms": round(kernel_ms / max(1, len(launch_ms)), 4),
         "compulsory_bytes_per_sweep": 2 * size * 8,
         "compulsory_GBs": round(2 * size * 8 / (kernel_ms * 1e-3) / 1e9, 1),
+        "note": ("achieved = reference flop count of one sweep / summed launch time "
+                 "of the sweep kernel (HIP events on the sweep stream); traffic = "
+                 "rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1 (copy-calibrated) per launch"),
     }
+
+    mixed = {}
+    for f in forms[1:]:
+        mixed[f] = {
+            "value": round(res[f]["value"], 3),
+            "ms_per_step": round(res[f]["ms_per_step"], 3),
+            "speedup_vs_fp64": round(res[f]["value"] / res[head]["value"], 3),
+            "latency_ms": {"p50": round(pct(res[f]["lat"], 50), 3),
+                           "p99": round(pct(res[f]["lat"], 99), 3)},
+            "normsq_rel_err_manufactured": abs(res[f]["normsq"] - expect) / expect,
+            "vo_normwise_err_vs_fp64_perturbed": errs[f]["vo_normwise"],
+            "normsq_rel_err_vs_fp64_perturbed": errs[f]["normsq_rel"],
+        }
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -194,35 +247,39 @@ def main():
 
     out = {
         "metric": METRIC,
-        "value": round(value, 3),
+        "value": round(r["value"], 3),
         "unit": "GFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "ms_per_step": round(r["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if head == "fp64" else f"{head} contractions, f32 faces, f64 state",
         "data": "synthetic: deterministic Quantities_init_* state (manufactured problem)",
         "config": {
-            "workload": (f"plain FP64 sweep, {n}^3 cells, ne={args.ne}, na={args.na}, "
-                         f"nm={args.nm}, nu={args.nu}, 8 octants, state resident in HBM"),
+            "workload": (f"{'plain FP64' if head == 'fp64' else 'mixed ' + head} sweep, "
+                         f"{n}^3 cells, ne={args.ne}, na={args.na}, nm={args.nm}, "
+                         f"nu={args.nu}, 8 octants, state resident in HBM"),
             "ncell": [n, n, n], "ne": args.ne, "na": args.na, "nm": args.nm,
-            "nu": args.nu, "precision": args.precision, "engine": args.engine,
+            "nu": args.nu, "precision": head, "engine": args.engine,
             "state_GB_each": round(size * 8 / 1e9, 3),
         },
-        "latency_ms": {"p50": round(pct(50), 3), "p99": round(pct(99), 3)},
-        "exact": exact,
-        "normsq_rel_err": normsq_rel_err,
-        "normsq": normsq,
+        "latency_ms": {"p50": round(pct(r["lat"], 50), 3), "p99": round(pct(r["lat"], 99), 3)},
+        "exact": r["normsqdiff"] == 0.0 and r["normsq"] == expect,
+        "normsq_rel_err": abs(r["normsq"] - expect) / expect,
+        "mixed": mixed or None,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     if args.launch_times:
-        print("launch_ms " + " ".join(f"{t:.3f}" for t in launch_ms), file=sys.stderr)
-    plan.close()
+        for f in forms:
+            print(f"launch_ms {f} " + " ".join(f"{t:.3f}" for t in res[f]["launch_ms"]),
+                  file=sys.stderr)
+    for p in plans.values():
+        p.close()
     if args.copy_probe_gib > 0:
         del vi, vo
         torch.cuda.empty_cache()

@@ -477,36 +477,30 @@ __global__ __launch_bounds__(64 * NWA) __attribute__((amdgpu_waves_per_eu(
 // Values are Quantities_init_face's divided by (1 + octant), angles in the
 // lane order of the sweep kernel.
 template <class PR>
-__global__ void k_fill_faces(SweepParams p) {
+__global__ __launch_bounds__(256) void k_fill_faces(SweepParams p) {
   typedef typename PR::T T;
+  // one workgroup per face row: (octant, group, unknown, y) of the x planes,
+  // then (octant, group, unknown, x) of the y planes; nz * na_pad values each
   const int nap = p.na_pad, nz = p.nz;
-  const size_t per_ouy_x = (size_t)p.ny * nz * nap;
-  const size_t per_ouy_y = (size_t)p.nx * nz * nap;
-  const size_t n_x = (size_t)kNoct * p.ne * p.nu * per_ouy_x;
-  const size_t n = n_x + (size_t)kNoct * p.ne * p.nu * per_ouy_y;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const bool xface = i < n_x;
-    size_t k = xface ? i : i - n_x;
-    const int pos = (int)(k % nap);
-    k /= nap;
-    const int z = (int)(k % nz);
-    k /= nz;
-    const int ext = xface ? p.ny : p.nx;
-    const int c = (int)(k % ext);  // y for x faces, x for y faces
-    k /= ext;
-    const int iu = (int)(k % p.nu);
-    k /= p.nu;
-    const int ie = (int)(k % p.ne);
-    const int oct = (int)(k / p.ne);
+  const int nrow_x = kNoct * p.ne * p.nu * p.ny;
+  const bool xface = (int)blockIdx.x < nrow_x;
+  int k = xface ? (int)blockIdx.x : (int)blockIdx.x - nrow_x;
+  const int ext = xface ? p.ny : p.nx;
+  const int c = k % ext;  // y for x faces, x for y faces
+  k /= ext;
+  const int iu = k % p.nu;
+  k /= p.nu;
+  const int ie = k % p.ne;
+  const int oct = k / p.ne;
+  T* row = xface ? reinterpret_cast<T*>(p.fyz) + (size_t)blockIdx.x * nz * nap
+                 : reinterpret_cast<T*>(p.fxz) + (size_t)(blockIdx.x - nrow_x) * nz * nap;
+  const double eu = (double)(sf_energy(ie) * sf_unknown(iu));
+  for (int i = threadIdx.x; i < nz * nap; i += blockDim.x) {
+    const int z = i / nap, pos = i - z * nap;
     const int ia = 16 * (pos >> 4) + PR::row((pos & 15) >> 2, pos & 3);
     const int e = xface ? sf_space_exp((oct & 1) ? p.nx : -1, c, z)
                         : sf_space_exp(c, (oct & 2) ? p.ny : -1, z);
-    const double v = ia < p.na ? ldexp((double)((1 + ia) << (ia & 7)) * sf_energy(ie) *
-                                           sf_unknown(iu), e)
-                               : 0.0;
-    (xface ? reinterpret_cast<T*>(p.fyz) : reinterpret_cast<T*>(p.fxz))[i - (xface ? 0 : n_x)] =
-        (T)v;
+    row[i] = ia < p.na ? (T)ldexp((double)((1 + ia) << (ia & 7)) * eu, e) : (T)0;
   }
 }
 
@@ -562,8 +556,7 @@ void launch_mfma(int precision, const LaunchSpec& ls, const SweepParams& sp, hip
 }
 
 void launch_mfma_fill(int precision, const SweepParams& sp, hipStream_t st) {
-  const size_t n = (size_t)kNoct * sp.ne * sp.nu * (sp.nx + sp.ny) * sp.nz * sp.na_pad;
-  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+  const unsigned blocks = (unsigned)(kNoct * sp.ne * sp.nu * (sp.nx + sp.ny));
   if (precision == MSW_PREC_FP64)
     hipLaunchKernelGGL(k_fill_faces<PrecF64>, dim3(blocks), dim3(256), 0, st, sp);
   else
