@@ -68,7 +68,7 @@ enum {
 typedef struct {
   int precision;       /* MSW_PREC_* */
   int engine;          /* MSW_ENGINE_* */
-  int angles_per_wave; /* 0: chosen from na; MFMA: 16 or 32; VALU: 8, 16 or 32 */
+  int angles_per_wave; /* 0: chosen from na; MFMA: 16, 32 or 64; VALU: 8, 16 or 32 */
 } MswPlanOptions;
 
 void msw_plan_default_options(MswPlanOptions* opt);

@@ -41,6 +41,23 @@ class MswPlanOptions(ctypes.Structure):
                 ("angles_per_wave", ctypes.c_int)]
 
 
+class MswBatchOptions(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("schedule_kind", ctypes.c_int), ("stagger", ctypes.c_int)]
+
+
+class MswServerOptions(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("hbm_budget_bytes", ctypes.c_size_t),
+                ("max_active", ctypes.c_int), ("keep_outputs", ctypes.c_int)]
+
+
+class MswResult(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_longlong), ("normsq", ctypes.c_double),
+                ("normsqdiff", ctypes.c_double), ("t_submit", ctypes.c_double),
+                ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("status", ctypes.c_int)]
+
+
 class SweepError(RuntimeError):
     pass
 
@@ -80,6 +97,30 @@ def lib():
         L.msw_init_matrices_host.argtypes = [ctypes.c_int, ctypes.c_int,
                                              ctypes.POINTER(ctypes.c_double),
                                              ctypes.POINTER(ctypes.c_double)]
+        # batched and served sweeps (msw_batch.h)
+        L.msw_batch_default_options.argtypes = [ctypes.POINTER(MswBatchOptions)]
+        L.msw_batch_create.argtypes = [ctypes.c_int, dims_p, ctypes.POINTER(MswBatchOptions),
+                                       ctypes.POINTER(vp)]
+        L.msw_batch_set_problem.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
+        L.msw_batch_run.argtypes = [vp, vp]
+        L.msw_batch_autotune.argtypes = [vp, vp, ctypes.c_int]
+        L.msw_batch_set_class_nac.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.msw_batch_num_kernel_launches.argtypes = [vp]
+        L.msw_batch_workspace_bytes.argtypes = [vp]
+        L.msw_batch_workspace_bytes.restype = ctypes.c_size_t
+        L.msw_batch_describe.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
+        L.msw_batch_destroy.argtypes = [vp]
+        L.msw_server_default_options.argtypes = [ctypes.POINTER(MswServerOptions)]
+        L.msw_server_create.argtypes = [ctypes.POINTER(MswServerOptions), ctypes.POINTER(vp)]
+        L.msw_server_submit.argtypes = [vp, dims_p, ctypes.c_longlong,
+                                        ctypes.POINTER(ctypes.c_double)]
+        L.msw_server_poll.argtypes = [vp, ctypes.POINTER(MswResult), ctypes.c_int]
+        L.msw_server_fetch.argtypes = [vp, ctypes.c_longlong, ctypes.POINTER(ctypes.c_double)]
+        L.msw_server_drain.argtypes = [vp, ctypes.c_double]
+        L.msw_server_now.argtypes = [vp]
+        L.msw_server_now.restype = ctypes.c_double
+        L.msw_server_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+        L.msw_server_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -201,3 +242,121 @@ def init_state(d: MswDims, v, stream=None):
 def state_shape(d: MswDims):
     """Shape of the reference state layout v[iz][ie][iy][ix][iu][im]."""
     return (d.ncell_z, d.ne, d.ncell_y, d.ncell_x, d.nu, d.nm)
+
+
+class Batch:
+    """Many independent sweeps run together (msw_batch.h): one hipGraph replay
+    per run, kernel variants chosen per (nm, na) class, optionally autotuned."""
+
+    def __init__(self, dims_list, precision: int = PREC_MIXED_FP32, use_graph: bool = True,
+                 schedule_kind: int = 0, stagger: bool = True):
+        o = MswBatchOptions()
+        lib().msw_batch_default_options(ctypes.byref(o))
+        o.precision, o.use_graph = precision, int(use_graph)
+        o.schedule_kind, o.stagger = schedule_kind, int(stagger)
+        arr = (MswDims * len(dims_list))(*dims_list)
+        h = ctypes.c_void_p()
+        _check(lib().msw_batch_create(len(dims_list), arr, ctypes.byref(o), ctypes.byref(h)),
+               "msw_batch_create")
+        self._h = h
+        self.dims = list(dims_list)
+
+    def set_problem(self, i, vi, vo, a_from_m, m_from_a):
+        _check(lib().msw_batch_set_problem(self._h, i, _dptr(vi), _dptr(vo), _dptr(a_from_m),
+                                           _dptr(m_from_a)), "msw_batch_set_problem")
+
+    def run(self, stream=None):
+        _check(lib().msw_batch_run(self._h, _stream_ptr(stream)), "msw_batch_run")
+
+    def autotune(self, reps: int = 2, stream=None):
+        _check(lib().msw_batch_autotune(self._h, _stream_ptr(stream), reps), "msw_batch_autotune")
+
+    def set_class_nac(self, nm: int, na: int, angles_per_wave: int):
+        _check(lib().msw_batch_set_class_nac(self._h, nm, na, angles_per_wave),
+               "msw_batch_set_class_nac")
+
+    def describe(self) -> dict:
+        import json
+        buf = ctypes.create_string_buffer(1 << 16)
+        _check(lib().msw_batch_describe(self._h, buf, len(buf)), "msw_batch_describe")
+        return json.loads(buf.value.decode())
+
+    @property
+    def num_kernel_launches(self) -> int:
+        return int(lib().msw_batch_num_kernel_launches(self._h))
+
+    @property
+    def workspace_bytes(self) -> int:
+        return int(lib().msw_batch_workspace_bytes(self._h))
+
+    def close(self):
+        if self._h:
+            lib().msw_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request loop with continuous batching (msw_batch.h): submit problems,
+    poll completions (FP64 norms and server-clock timestamps)."""
+
+    def __init__(self, precision: int = PREC_MIXED_FP32, hbm_budget_bytes: int = 32 << 30,
+                 max_active: int = 512, keep_outputs: bool = False):
+        o = MswServerOptions()
+        lib().msw_server_default_options(ctypes.byref(o))
+        o.precision, o.hbm_budget_bytes = precision, hbm_budget_bytes
+        o.max_active, o.keep_outputs = max_active, int(keep_outputs)
+        h = ctypes.c_void_p()
+        _check(lib().msw_server_create(ctypes.byref(o), ctypes.byref(h)), "msw_server_create")
+        self._h = h
+
+    def submit(self, d: MswDims, tag: int, vi_host=None):
+        p = None
+        if vi_host is not None:
+            vi_host = np.ascontiguousarray(vi_host, dtype=np.float64)
+            assert vi_host.size == size_state(d)
+            p = vi_host.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        _check(lib().msw_server_submit(self._h, ctypes.byref(d), tag, p), "msw_server_submit")
+
+    def poll(self, max_results: int = 4096):
+        buf = (MswResult * max_results)()
+        n = lib().msw_server_poll(self._h, buf, max_results)
+        if n < 0:
+            _check(-n, "msw_server_poll")
+        return [{"tag": r.tag, "normsq": r.normsq, "normsqdiff": r.normsqdiff,
+                 "t_submit": r.t_submit, "t_admit": r.t_admit, "t_done": r.t_done,
+                 "status": r.status} for r in buf[:n]]
+
+    def fetch(self, tag: int, d: MswDims):
+        out = np.empty(size_state(d))
+        _check(lib().msw_server_fetch(self._h, tag, out.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double))), "msw_server_fetch")
+        return out
+
+    def drain(self, timeout_s: float = 0.0):
+        _check(lib().msw_server_drain(self._h, timeout_s), "msw_server_drain")
+
+    def now(self) -> float:
+        return float(lib().msw_server_now(self._h))
+
+    def stats(self) -> dict:
+        out = (ctypes.c_double * 4)()
+        _check(lib().msw_server_stats(self._h, out), "msw_server_stats")
+        return {"done": int(out[0]), "kernel_launches": int(out[1]), "ticks": int(out[2]),
+                "arena_peak_bytes": int(out[3])}
+
+    def close(self):
+        if self._h:
+            lib().msw_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -73,4 +73,16 @@ struct SweepParams {
   int ntx, nty;
 };
 
+// A launch that runs segments of several sweeps: segment k covers
+// workgroups [start[k], start[k+1]) (the last one up to the grid size) and is
+// launch-local work of problem prob[k], whose items start at item_base[k].
+// Passed by value as a kernel argument.
+constexpr int kMaxSegments = 128;
+struct MultiLaunch {
+  int nseg;
+  int start[kMaxSegments];
+  int prob[kMaxSegments];
+  int item_base[kMaxSegments];
+};
+
 }  // namespace msw

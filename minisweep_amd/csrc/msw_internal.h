@@ -4,8 +4,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <string>
+#include <vector>
 
 #include "msw_device.h"
+#include "../../include/msw_plan.h"
 
 namespace msw {
 
@@ -26,10 +29,36 @@ size_t mfma_lds_bytes(int precision, int nac, int nwa);
 // Writes the domain-boundary inflow into the MFMA kernels' face planes; runs
 // before the launches of every sweep.
 void launch_mfma_fill(int precision, const SweepParams& sp, hipStream_t st);
+// Several sweeps' launches in one kernel launch (one kernel variant: all
+// segments' problems share nm, nac, nwa and the precision).  probs is a
+// device array of the problems' parameters.
+void launch_mfma_multi(int precision, int nm, int nac, int nwa, unsigned grid,
+                       const SweepParams* probs, const MultiLaunch& ml, hipStream_t st);
+void launch_mfma_fill_multi(int precision, unsigned grid, const SweepParams* probs,
+                            const MultiLaunch& ml, hipStream_t st);
+// Workgroups of one sweep's boundary fill.
+int mfma_fill_rows(const SweepParams& sp);
 // Bytes of one face value of the MFMA kernels in a precision.
 size_t mfma_face_elem_bytes(int precision);
 // Tile of one wave of the MFMA kernels (cells along x and y).
 int mfma_tile_x();
 int mfma_tile_y();
+
+// Sets the calling thread's msw_last_error() text; returns code.
+int set_error(int code, const std::string& msg);
+// Default angles per wave of the MFMA kernels for a precision and na.
+int default_mfma_nac(int precision, int na);
+
+// Host list schedule of (octant, tile) visits for ntx x nty tiles (kind 0:
+// most-remaining octant first; 1: the reference's octant order).
+std::vector<std::vector<TileItem>> build_schedule(int ntx, int nty, int kind);
+
+// Small kernels of sweep_plan.hip.
+void launch_pack(const double* afm, const double* mfa, double* at, double* mf, int nm, int na,
+                 int na_pad, hipStream_t st);
+void launch_init_state(const MswDims& d, double* v, hipStream_t st);
+size_t norms_scratch_doubles();
+void launch_norms(const double* vi, const double* vo, size_t n, double* part, double* out2,
+                  hipStream_t st);
 
 }  // namespace msw

@@ -172,9 +172,11 @@ struct Occupancy {
   static constexpr int value = sizeof(typename PR::T) * NAC <= 128 ? 2 : 1;
 };
 
+// One workgroup's work: tile item `it` of a sweep, energy group ie, unknown
+// iu, angle chunk = wave index.
 template <class PR, int NM, int NAC, int NWA>
-__global__ __launch_bounds__(64 * NWA) __attribute__((amdgpu_waves_per_eu(
-    Occupancy<PR, NAC>::value))) void k_sweep_mfma(SweepParams p, int item_base) {
+__device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem it,
+                                           const int ie, const int iu) {
   typedef typename PR::T T;
   typedef typename PR::A A;
   typedef typename PR::acc4 acc4;
@@ -188,12 +190,6 @@ __global__ __launch_bounds__(64 * NWA) __attribute__((amdgpu_waves_per_eu(
   const int lane = threadIdx.x & 63;
   const int wave = NWA == 1 ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int j = lane & 15, g = lane >> 4;
-  int bid = blockIdx.x;
-  const int iu = bid % p.nu;
-  bid /= p.nu;
-  const int ie = bid % p.ne;
-  bid /= p.ne;
-  const TileItem it = p.items[item_base + bid];
   const int oct = it.octant;
   const int dx = (oct & 1) ? -1 : 1;
   const int dy = (oct & 2) ? -1 : 1;
@@ -470,21 +466,54 @@ __global__ __launch_bounds__(64 * NWA) __attribute__((amdgpu_waves_per_eu(
   }
 }
 
+// Single sweep: workgroup -> (item of this launch, group, unknown).
+template <class PR, int NM, int NAC, int NWA>
+__global__ __launch_bounds__(64 * NWA) __attribute__((amdgpu_waves_per_eu(
+    Occupancy<PR, NAC>::value))) void k_sweep_mfma(SweepParams p, int item_base) {
+  int bid = blockIdx.x;
+  const int iu = bid % p.nu;
+  bid /= p.nu;
+  const int ie = bid % p.ne;
+  bid /= p.ne;
+  sweep_tile<PR, NM, NAC, NWA>(p, p.items[item_base + bid], ie, iu);
+}
+
+// Many sweeps in one launch: segment k of the launch is one launch of the
+// schedule of problem prob[k] (its items from item_base[k]); workgroups
+// [start[k], start[k+1]) belong to it.
+template <class PR, int NM, int NAC, int NWA>
+__global__ __launch_bounds__(64 * NWA) __attribute__((amdgpu_waves_per_eu(
+    Occupancy<PR, NAC>::value))) void k_sweep_mfma_multi(const SweepParams* __restrict__ probs,
+                                                       MultiLaunch ml) {
+  const int blk = (int)blockIdx.x;
+  int k = 0;
+  for (int step = kMaxSegments / 2; step > 0; step >>= 1)
+    if (k + step < ml.nseg && ml.start[k + step] <= blk) k += step;
+  const SweepParams p = probs[ml.prob[k]];
+  int bid = blk - ml.start[k];
+  const int iu = bid % p.nu;
+  bid /= p.nu;
+  const int ie = bid % p.ne;
+  bid /= p.ne;
+  sweep_tile<PR, NM, NAC, NWA>(p, p.items[ml.item_base[k] + bid], ie, iu);
+}
+
 // Boundary inflow of every tile row and column: before the sweep, the x face
 // plane of each (octant, group, unknown) holds the x-boundary value of every
 // (y, z, angle) and the y face plane the y-boundary value of every (x, z,
 // angle); the first tile of a row (column) reads them like any other inflow.
 // Values are Quantities_init_face's divided by (1 + octant), angles in the
 // lane order of the sweep kernel.
+// Face row `row` of a sweep: rows [0, 8 ne nu ny) are (octant, group,
+// unknown, y) of the x planes, the rest (octant, group, unknown, x) of the
+// y planes; nz * na_pad values each.
 template <class PR>
-__global__ __launch_bounds__(256) void k_fill_faces(SweepParams p) {
+__device__ __forceinline__ void fill_row(const SweepParams& p, int row) {
   typedef typename PR::T T;
-  // one workgroup per face row: (octant, group, unknown, y) of the x planes,
-  // then (octant, group, unknown, x) of the y planes; nz * na_pad values each
   const int nap = p.na_pad, nz = p.nz;
   const int nrow_x = kNoct * p.ne * p.nu * p.ny;
-  const bool xface = (int)blockIdx.x < nrow_x;
-  int k = xface ? (int)blockIdx.x : (int)blockIdx.x - nrow_x;
+  const bool xface = row < nrow_x;
+  int k = xface ? row : row - nrow_x;
   const int ext = xface ? p.ny : p.nx;
   const int c = k % ext;  // y for x faces, x for y faces
   k /= ext;
@@ -492,16 +521,33 @@ __global__ __launch_bounds__(256) void k_fill_faces(SweepParams p) {
   k /= p.nu;
   const int ie = k % p.ne;
   const int oct = k / p.ne;
-  T* row = xface ? reinterpret_cast<T*>(p.fyz) + (size_t)blockIdx.x * nz * nap
-                 : reinterpret_cast<T*>(p.fxz) + (size_t)(blockIdx.x - nrow_x) * nz * nap;
+  T* dst = xface ? reinterpret_cast<T*>(p.fyz) + (size_t)row * nz * nap
+                 : reinterpret_cast<T*>(p.fxz) + (size_t)(row - nrow_x) * nz * nap;
   const double eu = (double)(sf_energy(ie) * sf_unknown(iu));
   for (int i = threadIdx.x; i < nz * nap; i += blockDim.x) {
     const int z = i / nap, pos = i - z * nap;
     const int ia = 16 * (pos >> 4) + PR::row((pos & 15) >> 2, pos & 3);
     const int e = xface ? sf_space_exp((oct & 1) ? p.nx : -1, c, z)
                         : sf_space_exp(c, (oct & 2) ? p.ny : -1, z);
-    row[i] = ia < p.na ? (T)ldexp((double)((1 + ia) << (ia & 7)) * eu, e) : (T)0;
+    dst[i] = ia < p.na ? (T)ldexp((double)((1 + ia) << (ia & 7)) * eu, e) : (T)0;
   }
+}
+
+template <class PR>
+__global__ __launch_bounds__(256) void k_fill_faces(SweepParams p) {
+  fill_row<PR>(p, (int)blockIdx.x);
+}
+
+// Fill for several sweeps: segment k's workgroups are problem prob[k]'s rows.
+template <class PR>
+__global__ __launch_bounds__(256) void k_fill_faces_multi(const SweepParams* __restrict__ probs,
+                                                          MultiLaunch ml) {
+  const int blk = (int)blockIdx.x;
+  int k = 0;
+  for (int step = kMaxSegments / 2; step > 0; step >>= 1)
+    if (k + step < ml.nseg && ml.start[k + step] <= blk) k += step;
+  const SweepParams p = probs[ml.prob[k]];
+  fill_row<PR>(p, blk - ml.start[k]);
 }
 
 template <class PR, int NM, int NAC>
@@ -556,12 +602,78 @@ void launch_mfma(int precision, const LaunchSpec& ls, const SweepParams& sp, hip
 }
 
 void launch_mfma_fill(int precision, const SweepParams& sp, hipStream_t st) {
-  const unsigned blocks = (unsigned)(kNoct * sp.ne * sp.nu * (sp.nx + sp.ny));
+  const unsigned blocks = (unsigned)mfma_fill_rows(sp);
   if (precision == MSW_PREC_FP64)
     hipLaunchKernelGGL(k_fill_faces<PrecF64>, dim3(blocks), dim3(256), 0, st, sp);
   else
     hipLaunchKernelGGL(k_fill_faces<PrecF32>, dim3(blocks), dim3(256), 0, st, sp);
 }
+
+template <class PR, int NM, int NAC>
+static void launch_multi_nac(int nwa, unsigned grid, size_t lds, const SweepParams* probs,
+                             const MultiLaunch& ml, hipStream_t st) {
+  const dim3 block(64 * nwa);
+  switch (nwa) {
+    case 1:
+      hipLaunchKernelGGL((k_sweep_mfma_multi<PR, NM, NAC, 1>), dim3(grid), block, lds, st,
+                         probs, ml);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_sweep_mfma_multi<PR, NM, NAC, 2>), dim3(grid), block, lds, st,
+                         probs, ml);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_sweep_mfma_multi<PR, NM, NAC, 3>), dim3(grid), block, lds, st,
+                         probs, ml);
+      break;
+    default:
+      hipLaunchKernelGGL((k_sweep_mfma_multi<PR, NM, NAC, 4>), dim3(grid), block, lds, st,
+                         probs, ml);
+      break;
+  }
+}
+
+template <class PR, int NM>
+static void launch_multi_nm(int nac, int nwa, unsigned grid, size_t lds,
+                            const SweepParams* probs, const MultiLaunch& ml, hipStream_t st) {
+  if (nac == 16)
+    launch_multi_nac<PR, NM, 16>(nwa, grid, lds, probs, ml, st);
+  else if (nac == 32)
+    launch_multi_nac<PR, NM, 32>(nwa, grid, lds, probs, ml, st);
+  else
+    launch_multi_nac<PR, NM, 64>(nwa, grid, lds, probs, ml, st);
+}
+
+template <class PR>
+static void launch_multi_prec(int nm, int nac, int nwa, unsigned grid, size_t lds,
+                              const SweepParams* probs, const MultiLaunch& ml, hipStream_t st) {
+  switch (nm) {
+    case 4: launch_multi_nm<PR, 4>(nac, nwa, grid, lds, probs, ml, st); break;
+    case 8: launch_multi_nm<PR, 8>(nac, nwa, grid, lds, probs, ml, st); break;
+    default: launch_multi_nm<PR, 16>(nac, nwa, grid, lds, probs, ml, st); break;
+  }
+}
+
+void launch_mfma_multi(int precision, int nm, int nac, int nwa, unsigned grid,
+                       const SweepParams* probs, const MultiLaunch& ml, hipStream_t st) {
+  const size_t lds = mfma_lds_bytes(precision, nac, nwa);
+  if (precision == MSW_PREC_MIXED_FP32)
+    launch_multi_prec<PrecF32>(nm, nac, nwa, grid, lds, probs, ml, st);
+  else if (precision == MSW_PREC_MIXED_BF16)
+    launch_multi_prec<PrecBF16>(nm, nac, nwa, grid, lds, probs, ml, st);
+  else
+    launch_multi_prec<PrecF64>(nm, nac, nwa, grid, lds, probs, ml, st);
+}
+
+void launch_mfma_fill_multi(int precision, unsigned grid, const SweepParams* probs,
+                            const MultiLaunch& ml, hipStream_t st) {
+  if (precision == MSW_PREC_FP64)
+    hipLaunchKernelGGL(k_fill_faces_multi<PrecF64>, dim3(grid), dim3(256), 0, st, probs, ml);
+  else
+    hipLaunchKernelGGL(k_fill_faces_multi<PrecF32>, dim3(grid), dim3(256), 0, st, probs, ml);
+}
+
+int mfma_fill_rows(const SweepParams& sp) { return kNoct * sp.ne * sp.nu * (sp.nx + sp.ny); }
 
 size_t mfma_face_elem_bytes(int precision) {
   return precision == MSW_PREC_FP64 ? sizeof(double) : sizeof(float);

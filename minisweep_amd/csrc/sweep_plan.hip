@@ -48,6 +48,23 @@ static int set_err(int code, const char* fmt, const char* a = "", long b = 0) {
   return code;
 }
 
+int set_error(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+// Default angles per wave of the MFMA kernels: measured at 64^3, na=64, 32
+// per wave is fastest in FP64 (16 loses more to the cross-wave vo reduction
+// than it gains in occupancy) and 64 in the mixed forms; never more than na
+// rounded up to a 16-angle block, and at most 4 waves per tile.
+int default_mfma_nac(int precision, int na) {
+  const int cap = na <= 16 ? 16 : na <= 32 ? 32 : 64;
+  int nac = precision == MSW_PREC_FP64 ? 32 : 64;
+  nac = nac < cap ? nac : cap;
+  while ((na + nac - 1) / nac > 4 && nac < 64) nac *= 2;
+  return nac;
+}
+
 #define HIP_TRY(call)                                                        \
   do {                                                                       \
     hipError_t e_ = (call);                                                  \
@@ -401,9 +418,12 @@ __global__ __launch_bounds__(kNormThreads) void k_norms_final(
 //
 // Octant o's tile (i, j) in its own frame needs (i-1, j) and (i, j-1) done.
 // A launch takes every ready visit whose physical tile is not yet taken in
-// that launch, octants with the most remaining visits first.  Returns the
-// launches in order; marks the first visit of every physical tile.
-std::vector<std::vector<TileItem>> build_schedule(int ntx, int nty) {
+// that launch.  Octant priority: kind 0, most remaining visits first; kind 1,
+// the reference's fixed octant order {0,4,2,6,3,7,1,5}
+// (src/3_sweeper/stepscheduler_kba.c:155).  Returns the launches in order;
+// marks the first visit of every physical tile.
+std::vector<std::vector<TileItem>> build_schedule(int ntx, int nty, int kind) {
+  static const int kRefOrder[kNoct] = {0, 4, 2, 6, 3, 7, 1, 5};
   std::vector<std::vector<TileItem>> launches;
   std::vector<std::vector<int>> done(kNoct, std::vector<int>(ntx * nty, 0));
   std::vector<int> remaining(kNoct, ntx * nty);
@@ -411,9 +431,10 @@ std::vector<std::vector<TileItem>> build_schedule(int ntx, int nty) {
   int left = kNoct * ntx * nty;
   while (left > 0) {
     std::vector<int> order(kNoct);
-    for (int o = 0; o < kNoct; ++o) order[o] = o;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int a, int b) { return remaining[a] > remaining[b]; });
+    for (int o = 0; o < kNoct; ++o) order[o] = kind == 1 ? kRefOrder[o] : o;
+    if (kind != 1)
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int a, int b) { return remaining[a] > remaining[b]; });
     std::vector<int> used(ntx * nty, 0);
     std::vector<TileItem> items;
     for (int o : order) {
@@ -452,6 +473,29 @@ std::vector<std::vector<TileItem>> build_schedule(int ntx, int nty) {
     launches.push_back(items);
   }
   return launches;
+}
+
+void launch_pack(const double* afm, const double* mfa, double* at, double* mf, int nm, int na,
+                 int na_pad, hipStream_t st) {
+  const int n = kNoct * nm * na;
+  hipLaunchKernelGGL(k_pack_matrices, dim3((n + 255) / 256), dim3(256), 0, st, afm, mfa, at, mf,
+                     nm, na, na_pad);
+}
+
+void launch_init_state(const MswDims& d, double* v, hipStream_t st) {
+  const size_t n = (size_t)d.ncell_x * d.ncell_y * d.ncell_z * d.ne * d.nm * d.nu;
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_init_state, dim3(blocks), dim3(256), 0, st, v, d.ncell_x, d.ncell_y,
+                     d.ncell_z, d.ne, d.nm, d.nu, n);
+}
+
+size_t norms_scratch_doubles() { return 2 * (size_t)kNormBlocks; }
+
+void launch_norms(const double* vi, const double* vo, size_t n, double* part, double* out2,
+                  hipStream_t st) {
+  hipLaunchKernelGGL(k_norms_partial, dim3(kNormBlocks), dim3(kNormThreads), 0, st, vi, vo, n,
+                     part);
+  hipLaunchKernelGGL(k_norms_final, dim3(1), dim3(kNormThreads), 0, st, part, out2);
 }
 
 }  // namespace msw
@@ -581,11 +625,8 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
   p->engine = opt->engine;
   if (p->engine == MSW_ENGINE_MFMA) {
     // 16-angle MFMA blocks: 16, 32 or 64 angles per wave, at most 4 waves
-    // (angle chunks) per cell tile.  Measured at 64^3, na=64: 32 angles per wave is fastest in FP64 (16
-    // loses more to the cross-wave vo reduction than it gains in occupancy),
-    // 64 in the mixed forms
-    const int def = precision == MSW_PREC_FP64 ? (d->na <= 128 ? 32 : 64) : 64;
-    p->nac = opt->angles_per_wave ? opt->angles_per_wave : def;
+    // (angle chunks) per cell tile
+    p->nac = opt->angles_per_wave ? opt->angles_per_wave : default_mfma_nac(precision, d->na);
     if (p->nac != 16 && p->nac != 32 && p->nac != 64) {
       delete p;
       return set_err(MSW_ERR_INVALID,
@@ -616,7 +657,7 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
   else
     p->lds_bytes = sizeof(double) * ((size_t)p->nwa * 2 * kTile * (p->nac + 1) +
                                      (p->nwa > 1 ? (size_t)p->nwa * d->nm * 64 : 0));
-  auto launches = build_schedule(p->ntx, p->nty);
+  auto launches = build_schedule(p->ntx, p->nty, 0);
   std::vector<TileItem> flat;
   for (auto& l : launches) {
     p->launch_off.push_back((int)flat.size());

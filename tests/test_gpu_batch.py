@@ -1,0 +1,207 @@
+"""GPU tests of the batched and served forms (msw_batch.h through the C-ABI).
+
+Every problem's result must be bit-identical to its own single sweep
+(msw_plan_sweep with the same precision and angles per wave): the combined
+launches run the same kernels on the same tile schedule.  The single sweep is
+itself parity-checked against the oracle (test_gpu_fp64.py); here a few FP64
+problems are also checked against the oracle directly.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import minisweep_amd as msw
+from minisweep_amd.workloads import uneven_problems
+from parity import assert_close, closed_form_normsq
+
+pytestmark = pytest.mark.gpu
+
+# small uneven shapes (ncx, ncy, ncz, ne, nm, na, nu): partial tiles in x and y,
+# both nm classes, na below, at and above one 16-angle block multiple
+SHAPES = [
+    (9, 7, 3, 2, 16, 64, 4),
+    (17, 3, 5, 1, 4, 32, 2),
+    (3, 20, 9, 3, 4, 13, 3),
+    (16, 16, 4, 2, 16, 32, 4),
+    (5, 5, 5, 1, 8, 40, 1),
+    (20, 9, 6, 2, 16, 64, 4),
+]
+
+
+def _inputs(torch, dev, shapes, seed=7, perturb=True):
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in shapes:
+        d = msw.dims(*s)
+        base = oracle.init_state(oracle.dims(*s))
+        vi0 = base * (1.0 + (0.25 * rng.uniform(-1, 1, base.size) if perturb else 0.0))
+        a, m = msw.init_matrices(d.nm, d.na)
+        out.append((d, vi0, torch.from_numpy(vi0).to(dev),
+                    torch.full((base.size,), float("nan"), dtype=torch.float64, device=dev),
+                    torch.from_numpy(a).to(dev), torch.from_numpy(m).to(dev)))
+    return out
+
+
+def _single(torch, d, vi, a, m, prec):
+    vo = torch.full_like(vi, float("nan"))
+    plan = msw.Plan(d, prec, engine=msw.ENGINE_MFMA)
+    plan.sweep(vi, vo, a, m)
+    torch.cuda.synchronize()
+    plan.close()
+    return vo.cpu().numpy()
+
+
+@pytest.mark.parametrize("prec", [msw.PREC_FP64, msw.PREC_MIXED_FP32, msw.PREC_MIXED_BF16],
+                         ids=["fp64", "fp32", "bf16"])
+@pytest.mark.parametrize("graph", [True, False], ids=["graph", "eager"])
+def test_batch_matches_single_sweeps(prec, graph, cuda):
+    import torch
+    print(f"case batch prec {prec} graph {graph}", flush=True)
+    ins = _inputs(torch, cuda, SHAPES)
+    b = msw.Batch([x[0] for x in ins], precision=prec, use_graph=graph)
+    for i, (d, vi0, vi, vo, a, m) in enumerate(ins):
+        b.set_problem(i, vi, vo, a, m)
+    b.run()
+    torch.cuda.synchronize()
+    for d, vi0, vi, vo, a, m in ins:
+        np.testing.assert_array_equal(vo.cpu().numpy(), _single(torch, d, vi, a, m, prec))
+    if prec == msw.PREC_FP64:
+        for d, vi0, vi, vo, a, m in ins[:3]:
+            od = oracle.dims(*d.tuple())
+            oa, om = oracle.init_matrices(od)
+            assert_close(vo.cpu().numpy(), oracle.sweep(od, oa, om, vi0), what="batch vs oracle")
+    b.close()
+
+
+def test_batch_graph_replays_match_eager(cuda):
+    """Replays of the captured graph, with eager work and syncs between them,
+    each equal the eager run."""
+    import torch
+    print("case graph replays", flush=True)
+    ins = _inputs(torch, cuda, SHAPES, seed=11)
+    eager = msw.Batch([x[0] for x in ins], precision=msw.PREC_MIXED_FP32, use_graph=False)
+    graph = msw.Batch([x[0] for x in ins], precision=msw.PREC_MIXED_FP32, use_graph=True)
+    outs_e = [torch.full_like(x[2], float("nan")) for x in ins]
+    for i, (d, vi0, vi, vo, a, m) in enumerate(ins):
+        eager.set_problem(i, vi, outs_e[i], a, m)
+        graph.set_problem(i, vi, vo, a, m)
+    eager.run()
+    torch.cuda.synchronize()
+    ref = [o.cpu().numpy() for o in outs_e]
+    for rep in range(3):
+        for x in ins:
+            x[3].fill_(float("nan"))  # eager work between replays
+        torch.cuda.synchronize()
+        graph.run()
+        torch.cuda.synchronize()
+        for i, x in enumerate(ins):
+            np.testing.assert_array_equal(x[3].cpu().numpy(), ref[i])
+    eager.close()
+    graph.close()
+
+
+def test_batch_autotune_keeps_results(cuda):
+    import torch
+    print("case autotune", flush=True)
+    ins = _inputs(torch, cuda, SHAPES, perturb=False)  # manufactured: vo == vi exactly
+    b = msw.Batch([x[0] for x in ins], precision=msw.PREC_MIXED_FP32)
+    for i, (d, vi0, vi, vo, a, m) in enumerate(ins):
+        b.set_problem(i, vi, vo, a, m)
+    b.autotune(reps=1)
+    desc = b.describe()
+    assert desc["tuning"] and desc["tuning"][0]["try"] == "start"
+    for x in ins:
+        x[3].fill_(float("nan"))
+    b.run()
+    torch.cuda.synchronize()
+    for d, vi0, vi, vo, a, m in ins:
+        np.testing.assert_array_equal(vo.cpu().numpy(), vi0)
+    assert b.num_kernel_launches > 0
+    b.close()
+
+
+def test_batch_uneven_set_is_exact(cuda):
+    """A slice of the uneven problem set (seed 0), manufactured input."""
+    import torch
+    print("case uneven slice", flush=True)
+    shapes = [s for s in uneven_problems(256, 0) if s[0] * s[1] * s[2] * s[3] <= 60000][:12]
+    ins = _inputs(torch, cuda, shapes, perturb=False)
+    b = msw.Batch([x[0] for x in ins], precision=msw.PREC_MIXED_FP32)
+    for i, (d, vi0, vi, vo, a, m) in enumerate(ins):
+        b.set_problem(i, vi, vo, a, m)
+    b.run()
+    torch.cuda.synchronize()
+    for (d, vi0, vi, vo, a, m), s in zip(ins, shapes):
+        n, nd = msw.state_norms(d, vi, vo)
+        assert nd == 0.0 and n == float(closed_form_normsq(s[0], s[1], s[2], s[3], s[4], s[6]))
+    b.close()
+
+
+def test_server_results_alone_and_in_mix(cuda):
+    """Each request's vo is the same bits alone (one slot) and in a mix of
+    concurrently admitted requests, and equals its single sweep."""
+    import torch
+    print("case server alone vs mix", flush=True)
+    rng = np.random.default_rng(3)
+    shapes = SHAPES + [(8, 8, 8, 2, 16, 32, 4), (12, 5, 7, 1, 4, 64, 4)]
+    inputs = []
+    for s in shapes:
+        base = oracle.init_state(oracle.dims(*s))
+        inputs.append(base * (1.0 + 0.25 * rng.uniform(-1, 1, base.size)))
+    alone = {}
+    srv = msw.Server(precision=msw.PREC_MIXED_FP32, hbm_budget_bytes=1 << 30, max_active=1,
+                     keep_outputs=True)
+    for t, (s, v) in enumerate(zip(shapes, inputs)):
+        srv.submit(msw.dims(*s), t, v)
+        srv.drain(60)
+        alone[t] = srv.fetch(t, msw.dims(*s))
+    res = srv.poll()
+    assert sorted(r["tag"] for r in res) == list(range(len(shapes)))
+    assert all(r["status"] == 0 for r in res)
+    srv.close()
+    mix = msw.Server(precision=msw.PREC_MIXED_FP32, hbm_budget_bytes=1 << 30, max_active=64,
+                     keep_outputs=True)
+    for t, (s, v) in enumerate(zip(shapes, inputs)):
+        mix.submit(msw.dims(*s), t, v)
+    mix.drain(60)
+    res = mix.poll()
+    assert len(res) == len(shapes) and all(r["status"] == 0 for r in res)
+    for t, s in enumerate(shapes):
+        np.testing.assert_array_equal(mix.fetch(t, msw.dims(*s)), alone[t])
+    st = mix.stats()
+    assert st["done"] == len(shapes) and st["ticks"] > 0
+    mix.close()
+    # and equal to the single sweep
+    d = msw.dims(*shapes[0])
+    a, m = msw.init_matrices(d.nm, d.na)
+    vi = torch.from_numpy(inputs[0]).to(cuda)
+    ref = _single(torch, d, vi, torch.from_numpy(a).to(cuda), torch.from_numpy(m).to(cuda),
+                  msw.PREC_MIXED_FP32)
+    np.testing.assert_array_equal(alone[0], ref)
+
+
+def test_server_admission_by_footprint(cuda):
+    """A budget that holds only a few problems: requests wait for memory, all
+    finish, norms exact (manufactured input), arena peak within budget."""
+    print("case server footprint", flush=True)
+    budget = 64 << 20
+    srv = msw.Server(precision=msw.PREC_FP64, hbm_budget_bytes=budget, max_active=64)
+    shapes = [(16, 8, 8, 4, 16, 32, 4)] * 10 + [(8, 8, 4, 2, 4, 64, 4)] * 6
+    for t, s in enumerate(shapes):
+        srv.submit(msw.dims(*s), t)
+    srv.drain(120)
+    res = sorted(srv.poll(), key=lambda r: r["tag"])
+    assert len(res) == len(shapes)
+    for r, s in zip(res, shapes):
+        assert r["status"] == 0
+        assert r["normsqdiff"] == 0.0
+        assert r["normsq"] == float(closed_form_normsq(s[0], s[1], s[2], s[3], s[4], s[6]))
+        assert r["t_submit"] <= r["t_admit"] <= r["t_done"]
+    st = srv.stats()
+    assert 0 < st["arena_peak_bytes"] <= budget
+    # a request larger than the whole budget fails cleanly
+    srv.submit(msw.dims(64, 64, 64, 8, 16, 64, 4), 999)
+    srv.drain(60)
+    r = srv.poll()
+    assert len(r) == 1 and r[0]["tag"] == 999 and r[0]["status"] == 4  # MSW_ERR_NOMEM
+    srv.close()

@@ -1,0 +1,24 @@
+"""Problem sets of the batched and served forms (host only)."""
+import numpy as np
+
+from minisweep_amd.workloads import poisson_arrivals, uneven_problems
+
+
+def test_uneven_problems_deterministic_and_in_range():
+    a = uneven_problems(256, 0)
+    assert a == uneven_problems(256, 0)
+    assert a != uneven_problems(256, 1)
+    assert len(a) == 256
+    for ncx, ncy, ncz, ne, nm, na, nu in a:
+        assert all(8 <= c <= 48 for c in (ncx, ncy, ncz))
+        assert ne in (8, 16, 32) and na in (32, 64) and nm in (4, 16) and nu == 4
+    # log-uniform per axis: the median is near sqrt(8*48) ~ 19.6
+    med = np.median([c for s in a for c in s[:3]])
+    assert 15 <= med <= 25
+    assert {s[4] for s in a} == {4, 16} and {s[5] for s in a} == {32, 64}
+
+
+def test_poisson_arrivals():
+    t = poisson_arrivals(10000, 200.0, 1)
+    assert np.all(np.diff(t) > 0)
+    assert abs(t[-1] / len(t) - 1 / 200.0) < 0.05 / 200.0 * 10
