@@ -95,7 +95,8 @@ def time_precision(msw, torch, d, prec_name, args, ta, tm, vi, vo, flops_sweep):
     prec = {"fp64": msw.PREC_FP64, "fp32": msw.PREC_MIXED_FP32,
             "bf16": msw.PREC_MIXED_BF16}[prec_name]
     engine = msw.ENGINE_MFMA if args.engine == "mfma" else msw.ENGINE_VALU
-    plan = msw.Plan(d, prec, engine=engine, angles_per_wave=args.angles_per_wave)
+    plan = msw.Plan(d, prec, engine=engine, angles_per_wave=args.angles_per_wave,
+                    groups_per_workgroup=args.groups_per_workgroup)
     stream = torch.cuda.current_stream()
     bufs = [vi, vo]
     for i in range(args.warmup):
@@ -141,6 +142,7 @@ def main():
                     help="time only this form (default: FP64 headline plus both "
                          "mixed forms)")
     ap.add_argument("--angles-per-wave", type=int, default=0)
+    ap.add_argument("--groups-per-workgroup", type=int, default=0)
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="threads for the CPU baseline (0: OMP_NUM_THREADS or 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -69,6 +69,8 @@ typedef struct {
   int precision;       /* MSW_PREC_* */
   int engine;          /* MSW_ENGINE_* */
   int angles_per_wave; /* 0: chosen from na; MFMA: 16, 32 or 64; VALU: 8, 16 or 32 */
+  int groups_per_workgroup; /* MFMA: energy groups streamed through one tile
+                               pipeline (0: chosen from ncell_z) */
 } MswPlanOptions;
 
 void msw_plan_default_options(MswPlanOptions* opt);

@@ -92,7 +92,7 @@ void free_schedules(ScheduleCache& c) {
 // Geometry of one problem under the MFMA kernels.
 struct Geom {
   MswDims d{};
-  int nac = 16, nwa = 1, nap = 16, ntx = 1, nty = 1;
+  int nac = 16, nwa = 1, nap = 16, ntx = 1, nty = 1, epw = 1;
   size_t mat_doubles() const { return ((size_t)kNoct * nap + nac) * d.nm; }
   size_t face_elems() const {
     return (size_t)kNoct * d.ne * d.nu * (d.ncell_x + d.ncell_y) * d.ncell_z * nap;
@@ -124,6 +124,7 @@ int make_geom(const MswDims& d, int nac, Geom* g) {
   g->nap = g->nwa * nac;
   g->ntx = (d.ncell_x + mfma_tile_x() - 1) / mfma_tile_x();
   g->nty = (d.ncell_y + mfma_tile_y() - 1) / mfma_tile_y();
+  g->epw = default_mfma_epw(d.ncell_z, d.ne);
   return MSW_OK;
 }
 
@@ -146,6 +147,7 @@ SweepParams make_params(const Geom& g, const double* vi, double* vo, const doubl
   sp.na_pad = g.nap;
   sp.ntx = g.ntx;
   sp.nty = g.nty;
+  sp.epw = g.epw;
   return sp;
 }
 
@@ -176,7 +178,8 @@ int enqueue_tick(int precision, const std::vector<TickWork>& work, const SweepPa
         ml.prob[ml.nseg] = w.slot;
         ml.item_base[ml.nseg] = w.s->off[w.launch];
         ml.nseg++;
-        grid += (long long)w.s->cnt[w.launch] * w.g->d.ne * w.g->d.nu;
+        grid += (long long)w.s->cnt[w.launch] * ((w.g->d.ne + w.g->epw - 1) / w.g->epw) *
+                w.g->d.nu;
       }
       const Geom& g = *ws[b]->g;
       launch_mfma_multi(precision, g.d.nm, g.nac, g.nwa, (unsigned)grid, d_params, ml, st);
@@ -250,6 +253,7 @@ struct MswBatch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int nkernels = 0, nticks = 0;
+  int stream_target = 128;  // z-stream steps per workgroup (energy groups streamed)
   std::string log;
 };
 
@@ -286,7 +290,7 @@ void stagger_starts(MswBatch* b) {
   for (size_t i = 0; i < b->probs.size(); ++i) {
     order[i] = (int)i;
     const BatchProb& p = b->probs[i];
-    for (int l = 0; l < p.s->nlaunch(); ++l) total[i] += (long long)p.s->cnt[l] * p.g.d.ne * p.g.d.nu;
+    for (int l = 0; l < p.s->nlaunch(); ++l) total[i] += (long long)p.s->cnt[l] * ((p.g.d.ne + p.g.epw - 1) / p.g.epw) * p.g.d.nu;
   }
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return total[x] > total[y]; });
   for (int i : order) {
@@ -297,14 +301,14 @@ void stagger_starts(MswBatch* b) {
     for (int s0 = 0; s0 + L <= horizon; ++s0) {
       long long peak = 0;
       for (int l = 0; l < L; ++l)
-        peak = std::max(peak, load[s0 + l] + (long long)p.s->cnt[l] * p.g.d.ne * p.g.d.nu);
+        peak = std::max(peak, load[s0 + l] + (long long)p.s->cnt[l] * ((p.g.d.ne + p.g.epw - 1) / p.g.epw) * p.g.d.nu);
       if (best_peak < 0 || peak < best_peak) {
         best_peak = peak;
         best = s0;
       }
     }
     p.start_tick = best;
-    for (int l = 0; l < L; ++l) load[best + l] += (long long)p.s->cnt[l] * p.g.d.ne * p.g.d.nu;
+    for (int l = 0; l < L; ++l) load[best + l] += (long long)p.s->cnt[l] * ((p.g.d.ne + p.g.epw - 1) / p.g.epw) * p.g.d.nu;
   }
 }
 
@@ -353,6 +357,8 @@ int batch_build(MswBatch* b) {
     BatchProb& p = b->probs[i];
     int rc = make_geom(b->dims[i], nac_for(b, b->dims[i]), &p.g);
     if (rc) return rc;
+    const int nz = p.g.d.ncell_z;
+    p.g.epw = std::max(1, std::min(p.g.d.ne, (b->stream_target + nz / 2) / nz));
     rc = get_schedule(b->scheds, p.g.ntx, p.g.nty, b->opt.schedule_kind, &p.s);
     if (rc) return rc;
     offs[i] = total;
@@ -539,6 +545,20 @@ int msw_batch_autotune(MswBatch* b, void* stream, int reps) {
     b->class_nac[c] = keep;
     b->built = false;
   }
+  for (int target : {64, 256}) {
+    const int save = b->stream_target;
+    b->stream_target = target;
+    b->built = false;
+    double ms = 0;
+    rc = batch_time(b, st, reps, &ms);
+    if (rc) return rc;
+    note("stream_target" + std::to_string(target), ms);
+    if (ms < best)
+      best = ms;
+    else
+      b->stream_target = save;
+    b->built = false;
+  }
   for (int kind : {0, 1})
     for (int stag : {0, 1}) {
       if (kind == b->opt.schedule_kind && stag == b->opt.stagger) continue;
@@ -571,6 +591,7 @@ int msw_batch_describe(const MswBatch* b, char* buf, int len) {
                   ", \"schedule_kind\": " + std::to_string(b->opt.schedule_kind) +
                   ", \"stagger\": " + std::to_string(b->opt.stagger) +
                   ", \"graph\": " + std::to_string(b->opt.use_graph) +
+                  ", \"stream_target\": " + std::to_string(b->stream_target) +
                   ", \"ticks\": " + std::to_string(b->nticks) +
                   ", \"kernels_per_run\": " + std::to_string(b->nkernels) + ", \"class_nac\": {";
   bool first = true;

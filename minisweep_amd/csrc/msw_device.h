@@ -65,12 +65,14 @@ struct SweepParams {
   double* vo;
   const double* at;  // a_from_m transposed: [8][na][nm]
   const double* mf;  // m_from_a:            [8][na][nm]
-  void* fyz;         // x-direction faces  [8][ne][nu][ny][nz][na] (grid frame)
-  void* fxz;         // y-direction faces  [8][ne][nu][nx][nz][na] (grid frame)
+  void* fyz;         // x-direction faces: VALU [8][ne][nu][ny][nz][na] (grid frame),
+                     //   MFMA [8][nu][ny][ne][nz][na] (octant-frame z)
+  void* fxz;         // y-direction faces: the same with nx for ny
   const TileItem* items;
   int nx, ny, nz, ne, na, nu;
   int na_pad;  // na rounded up to whole angle chunks (matrix/face stride)
   int ntx, nty;
+  int epw;     // energy groups streamed through one workgroup (MFMA kernels)
 };
 
 // A launch that runs segments of several sweeps: segment k covers

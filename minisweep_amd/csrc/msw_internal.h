@@ -48,6 +48,8 @@ int mfma_tile_y();
 int set_error(int code, const std::string& msg);
 // Default angles per wave of the MFMA kernels for a precision and na.
 int default_mfma_nac(int precision, int na);
+// Default energy groups per workgroup of the MFMA kernels.
+int default_mfma_epw(int nz, int ne);
 
 // Host list schedule of (octant, tile) visits for ntx x nty tiles (kind 0:
 // most-remaining octant first; 1: the reference's octant order).

@@ -172,11 +172,14 @@ struct Occupancy {
   static constexpr int value = sizeof(typename PR::T) * NAC <= 128 ? 2 : 1;
 };
 
-// One workgroup's work: tile item `it` of a sweep, energy group ie, unknown
-// iu, angle chunk = wave index.
+// One workgroup's work: tile item `it` of a sweep, energy groups
+// [ie0, ie0 + nge) streamed back to back through one z pipeline, unknown iu,
+// angle chunk = wave index.  Lane (j, b) is at stream position
+// t - j - b = (group - ie0) * nz + z, so the tile's fill and drain are paid
+// once per nge groups.
 template <class PR, int NM, int NAC, int NWA>
 __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem it,
-                                           const int ie, const int iu) {
+                                           const int ie0, const int nge, const int iu) {
   typedef typename PR::T T;
   typedef typename PR::A A;
   typedef typename PR::acc4 acc4;
@@ -213,9 +216,9 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
   int yb[kTY], cst[kTY];
   const size_t Se = (size_t)ny * nx * nu * NM;  // state stride of a group
   const size_t Sz = (size_t)p.ne * Se;          // state stride of a z plane
-  // state offset of the lane's cells at z = 0: a per-lane x part plus a
-  // per-block y part that is the same for all lanes
-  const size_t cxo = (size_t)ie * Se + ((size_t)x * nu + iu) * NM;
+  // state offset of the lane's cells at z = 0 of group ie0: a per-lane x part
+  // plus a per-block y part that is the same for all lanes
+  const size_t cxo = (size_t)ie0 * Se + ((size_t)x * nu + iu) * NM;
   size_t coff[kTY];
 #pragma unroll
   for (int b = 0; b < kTY; ++b) {
@@ -232,10 +235,14 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
   T* xs_lds = reinterpret_cast<T*>(red + (NWA > 1 ? 2 * kTY * NWA * 64 * 4 : 0)) +
               (size_t)wave * NAB * 64;
 
-  // ---- face planes of this (octant, group, unknown) ------------------------
-  const size_t face_ouy = ((size_t)oct * p.ne + ie) * nu + iu;
-  T* fyz = reinterpret_cast<T*>(p.fyz) + face_ouy * ny * nz * nap + ia0 + 4 * g;  // [y][z][ia]
-  T* fxz = reinterpret_cast<T*>(p.fxz) + face_ouy * nx * nz * nap + ia0 + 4 * g;  // [x][z][ia]
+  // ---- face planes of this (octant, unknown): [y or x][group][z][angle],
+  // z in the octant's frame so that a stream position maps to one offset --
+  const size_t nze = (size_t)p.ne * nz;  // stream length of all groups
+  const size_t face_ou = (size_t)oct * nu + iu;
+  T* fyz = reinterpret_cast<T*>(p.fyz) + (face_ou * ny * nze + (size_t)ie0 * nz) * nap + ia0 +
+           4 * g;
+  T* fxz = reinterpret_cast<T*>(p.fxz) + (face_ou * nx * nze + (size_t)ie0 * nz) * nap + ia0 +
+           4 * g;
 
   // ---- per-octant operands, in registers for the whole launch --------------
   const int moff = MSTEP * g;
@@ -264,7 +271,7 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
 #pragma unroll
     for (int s = 0; s < PR::kWz; ++s) wz[s] = (T)0.25 - ldexpT((T)1, -(PR::row(g, s) & 7));
   }
-  const T eu = (T)(sf_energy(ie) * sf_unknown(iu));
+  const int su = sf_unknown(iu);
   const unsigned lut_z = coef_lut(dz), lut_y = coef_lut(dy), lut_x = coef_lut(-dx);
 
   T F[NAB][kTY][4];  // outgoing face value (= solved value) of the lane's cells
@@ -283,17 +290,44 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
   double von[NOWN][4];   // old vo of the owned blocks' cells
   T xsn[NAB];            // x inflow, spread: lane (j, g) holds block j/4, register j%4
   T yinn[NAB][4];        // y inflow of block 0
-  const int nsteps = nz + kTX + kTY - 2;
+  const int ns = nge * nz;  // stream length of this workgroup
+  const int nsteps = ns + kTX + kTY - 2;
   const bool mpad = moff + KA > NM;  // packed operand past the last moment
 
-  auto zclamp = [&](int zp) {
-    const int zq = zp < 0 ? 0 : zp >= nz ? nz - 1 : zp;
-    return dz > 0 ? zq : nz - 1 - zq;
+  // Stream position of block b's cell: (group offset es, z in the octant's
+  // frame zs), advanced by one each step.  Before the stream zs < 0, after it
+  // es >= nge; addresses use the clamped position.
+  int zs[kTY], es[kTY];
+#pragma unroll
+  for (int b = 0; b < kTY; ++b) {
+    zs[b] = -j - b;
+    es[b] = 0;
+  }
+  // state offset of (group offset e, octant-frame z) in a block, clamped
+  auto soff = [&](int b, int e, int zq) {
+    const int zc = zq < 0 ? 0 : zq;
+    const int ec = e < nge ? e : nge - 1;
+    const int zg = dz > 0 ? zc : nz - 1 - zc;
+    return coff[b] + (size_t)ec * Se + (size_t)zg * Sz;
   };
-  auto fetch_vi = [&](int t) {
+  // face-plane offset of stream position sp, clamped
+  auto foff = [&](int sp) {
+    const int sc = sp < 0 ? 0 : sp >= ns ? ns - 1 : sp;
+    return (size_t)sc * nap;
+  };
+  int zn[kTY], en[kTY];  // next step's stream position per block
+  auto advance = [&]() {
 #pragma unroll
     for (int b = 0; b < kTY; ++b) {
-      const double* src = p.vi + coff[b] + (size_t)zclamp(t - j - b) * Sz + moff;
+      const bool wrap = zs[b] + 1 == nz;
+      zn[b] = wrap ? 0 : zs[b] + 1;
+      en[b] = wrap ? es[b] + 1 : es[b];
+    }
+  };
+  auto fetch_vi = [&](const int (&ze)[kTY], const int (&ee)[kTY]) {
+#pragma unroll
+    for (int b = 0; b < kTY; ++b) {
+      const double* src = p.vi + soff(b, ee[b], ze[b]) + moff;
       if constexpr (KA == 4) {
         double v[4] = {0.0, 0.0, 0.0, 0.0};
         if (!mpad) load4(src, v);
@@ -305,32 +339,33 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
       }
     }
   };
-  auto fetch_vo = [&](int t) {
+  auto fetch_vo = [&](const int (&ze)[kTY], const int (&ee)[kTY]) {
 #pragma unroll
     for (int b = 0; b < kTY; ++b)
       if (b % NWA == wave && 4 * g < NM)
-        load4(p.vo + coff[b] + (size_t)zclamp(t - j - b) * Sz + 4 * g, von[b / NWA]);
+        load4(p.vo + soff(b, ee[b], ze[b]) + 4 * g, von[b / NWA]);
   };
   auto fetch_xs = [&](int t) {
     const int bb = j >> 2;
     const int bc = bb < wy ? bb : wy - 1;
     const int y = dy > 0 ? gy * kTY + bc : gy * kTY + wy - 1 - bc;
-    const T* src = fyz + ((size_t)y * nz + zclamp(t - bb)) * nap + (j & 3);
+    const T* src = fyz + (size_t)y * nze * nap + foff(t - bb) + (j & 3);
 #pragma unroll
     for (int a = 0; a < NAB; ++a) xsn[a] = src[16 * a];
   };
   auto fetch_yin = [&](int t) {
-    const T* src = fxz + ((size_t)x * nz + zclamp(t - j)) * nap;
+    const T* src = fxz + (size_t)x * nze * nap + foff(t - j);
 #pragma unroll
     for (int a = 0; a < NAB; ++a) load4(src + 16 * a, yinn[a]);
   };
 
   fetch_xs(0);
-  fetch_vi(0);
+  fetch_vi(zs, es);
   fetch_yin(0);
-  fetch_vo(0);
+  fetch_vo(zs, es);
 
   for (int t = 0; t < nsteps; ++t) {
+    advance();
     // x inflow of this step into LDS: lane (0, g) reads its 16 values back
 #pragma unroll
     for (int a = 0; a < NAB; ++a) xs_lds[a * 64 + lane] = xsn[a];
@@ -340,21 +375,21 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
     fetch_xs(t + 1);
 
     // ---- per cell block: validity, z, spatial phase; z inflow on entry -------
-    bool val[kTY];
+    // A lane whose cell starts a group (z = 0) takes the domain-boundary z
+    // inflow instead of its own face register; the register itself still
+    // holds the previous group's last cell, which its x and y neighbours read
+    // as their inflow in this step.
+    bool val[kTY], ent[kTY];
     int zgs[kTY], sh[kTY];
+    size_t so[kTY];
 #pragma unroll
     for (int b = 0; b < kTY; ++b) {
-      const int zp = t - j - b;
-      val[b] = x_ok && b < wy && (unsigned)zp < (unsigned)nz;
-      zgs[b] = zclamp(zp);
+      val[b] = x_ok && b < wy && zs[b] >= 0 && es[b] < nge;
+      ent[b] = zs[b] == 0 && es[b] < nge;
+      const int zc = zs[b] < 0 ? 0 : zs[b];
+      zgs[b] = dz > 0 ? zc : nz - 1 - zc;
+      so[b] = soff(b, es[b], zs[b]);
       sh[b] = ((cst[b] + 7 * zgs[b]) & 3) * 4;
-      if (zp == 0) {
-        const T sc = ldexpT(eu, sf_space_exp(x, yb[b], zb));
-#pragma unroll
-        for (int a = 0; a < NAB; ++a)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) F[a][b][r] = (T)iaf[a][r] * sc;
-      }
     }
 
     acc4 vacc[kTY];
@@ -374,7 +409,7 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
           for (int s = 0; s < KM; ++s) vs[b] = PR::mma(aop[a][s], vin[b][s], vs[b]);
         }
       }
-      if (a == NAB - 1) fetch_vi(t + 1);
+      if (a == NAB - 1) fetch_vi(zn, en);
       // solve, blocks in descending order: block b's y face is block b-1's
       // value from the previous step, read before it is overwritten
 #pragma unroll
@@ -384,12 +419,22 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
         const T cz = ldexpT((T)0x1p-9, (int)((lut_z >> sh[b]) & 15));
         const T cy = ldexpT((T)0x1p-10, (int)((lut_y >> sh[b]) & 15));
         const int xe = (int)((lut_x >> sh[b]) & 15) - 8;
+        T rz[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) rz[r] = F[a][b][r];
+        if (__ballot(ent[b])) {  // some lane of this block starts a group
+          const T sc = ent[b] ? (T)ldexp((double)(sf_energy(ie0 + es[b]) * su),
+                                         sf_space_exp(x, yb[b], zb))
+                              : (T)0;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) rz[r] = ent[b] ? (T)iaf[a][r] * sc : rz[r];
+        }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const T rx = shr1_keep(xin[r], F[a][b][r]);
           const T ry = b > 0 ? F[a][b > 0 ? b - 1 : 0][r] : yinn[a][r];
           const T res = fmaT(rx, ldexpT(wz[r % PR::kWz], xe),
-                             fmaT(ry, cy, fmaT(F[a][b][r], cz, (T)vs[b][r])));
+                             fmaT(ry, cy, fmaT(rz[r], cz, (T)vs[b][r])));
           F[a][b][r] = res;
           vs[b][r] = res;
         }
@@ -413,7 +458,7 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
 #pragma unroll
       for (int b = 0; b < kTY; ++b)
         if (val[b]) {
-          T* dst = fyz + ((size_t)yb[b] * nz + zgs[b]) * nap;
+          T* dst = fyz + (size_t)yb[b] * nze * nap + foff(t - j - b);
 #pragma unroll
           for (int a = 0; a < NAB; ++a) store4(dst + 16 * a, F[a][b]);
         }
@@ -422,7 +467,7 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
 #pragma unroll
       for (int b = 0; b < kTY; ++b)
         if (b == wy - 1 && val[b]) {
-          T* dst = fxz + ((size_t)x * nz + zgs[b]) * nap;
+          T* dst = fxz + (size_t)x * nze * nap + foff(t - j - b);
 #pragma unroll
           for (int a = 0; a < NAB; ++a) store4(dst + 16 * a, F[a][b]);
         }
@@ -460,22 +505,34 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
 #pragma unroll
         for (int r = 0; r < 4; ++r) sum[r] += von[b / NWA][r];
       }
-      if (val[b] && 4 * g < NM) store4(p.vo + coff[b] + (size_t)zgs[b] * Sz + 4 * g, sum);
+      if (val[b] && 4 * g < NM) store4(p.vo + so[b] + 4 * g, sum);
     }
-    fetch_vo(t + 1);
+    fetch_vo(zn, en);
+#pragma unroll
+    for (int b = 0; b < kTY; ++b) {
+      zs[b] = zn[b];
+      es[b] = en[b];
+    }
   }
 }
 
-// Single sweep: workgroup -> (item of this launch, group, unknown).
+// Workgroup bid of a launch -> (item, chunk of p.epw energy groups, unknown).
+template <class PR, int NM, int NAC, int NWA>
+__device__ __forceinline__ void sweep_block(const SweepParams& p, int item_base, int bid) {
+  const int iu = bid % p.nu;
+  bid /= p.nu;
+  const int nec = (p.ne + p.epw - 1) / p.epw;
+  const int ec = bid % nec;
+  bid /= nec;
+  const int ie0 = ec * p.epw;
+  sweep_tile<PR, NM, NAC, NWA>(p, p.items[item_base + bid], ie0, min(p.epw, p.ne - ie0), iu);
+}
+
+// Single sweep.
 template <class PR, int NM, int NAC, int NWA>
 __global__ __launch_bounds__(64 * NWA) __attribute__((amdgpu_waves_per_eu(
     Occupancy<PR, NAC>::value))) void k_sweep_mfma(SweepParams p, int item_base) {
-  int bid = blockIdx.x;
-  const int iu = bid % p.nu;
-  bid /= p.nu;
-  const int ie = bid % p.ne;
-  bid /= p.ne;
-  sweep_tile<PR, NM, NAC, NWA>(p, p.items[item_base + bid], ie, iu);
+  sweep_block<PR, NM, NAC, NWA>(p, item_base, (int)blockIdx.x);
 }
 
 // Many sweeps in one launch: segment k of the launch is one launch of the
@@ -490,12 +547,7 @@ __global__ __launch_bounds__(64 * NWA) __attribute__((amdgpu_waves_per_eu(
   for (int step = kMaxSegments / 2; step > 0; step >>= 1)
     if (k + step < ml.nseg && ml.start[k + step] <= blk) k += step;
   const SweepParams p = probs[ml.prob[k]];
-  int bid = blk - ml.start[k];
-  const int iu = bid % p.nu;
-  bid /= p.nu;
-  const int ie = bid % p.ne;
-  bid /= p.ne;
-  sweep_tile<PR, NM, NAC, NWA>(p, p.items[ml.item_base[k] + bid], ie, iu);
+  sweep_block<PR, NM, NAC, NWA>(p, ml.item_base[k], blk - ml.start[k]);
 }
 
 // Boundary inflow of every tile row and column: before the sweep, the x face
@@ -504,32 +556,36 @@ __global__ __launch_bounds__(64 * NWA) __attribute__((amdgpu_waves_per_eu(
 // angle); the first tile of a row (column) reads them like any other inflow.
 // Values are Quantities_init_face's divided by (1 + octant), angles in the
 // lane order of the sweep kernel.
-// Face row `row` of a sweep: rows [0, 8 ne nu ny) are (octant, group,
-// unknown, y) of the x planes, the rest (octant, group, unknown, x) of the
-// y planes; nz * na_pad values each.
+// Face row `row` of a sweep: rows [0, 8 nu ny) are (octant, unknown, y) of
+// the x planes, the rest (octant, unknown, x) of the y planes; each holds
+// ne * nz * na_pad values ordered (group, octant-frame z, lane-order angle).
 template <class PR>
 __device__ __forceinline__ void fill_row(const SweepParams& p, int row) {
   typedef typename PR::T T;
   const int nap = p.na_pad, nz = p.nz;
-  const int nrow_x = kNoct * p.ne * p.nu * p.ny;
+  const int nrow_x = kNoct * p.nu * p.ny;
   const bool xface = row < nrow_x;
   int k = xface ? row : row - nrow_x;
   const int ext = xface ? p.ny : p.nx;
   const int c = k % ext;  // y for x faces, x for y faces
   k /= ext;
   const int iu = k % p.nu;
-  k /= p.nu;
-  const int ie = k % p.ne;
-  const int oct = k / p.ne;
-  T* dst = xface ? reinterpret_cast<T*>(p.fyz) + (size_t)row * nz * nap
-                 : reinterpret_cast<T*>(p.fxz) + (size_t)(row - nrow_x) * nz * nap;
-  const double eu = (double)(sf_energy(ie) * sf_unknown(iu));
-  for (int i = threadIdx.x; i < nz * nap; i += blockDim.x) {
-    const int z = i / nap, pos = i - z * nap;
+  const int oct = k / p.nu;
+  const int dz = (oct & 4) ? -1 : 1;
+  const size_t len = (size_t)p.ne * nz * nap;
+  T* dst = xface ? reinterpret_cast<T*>(p.fyz) + (size_t)row * len
+                 : reinterpret_cast<T*>(p.fxz) + (size_t)(row - nrow_x) * len;
+  const int su = sf_unknown(iu);
+  for (int i = threadIdx.x; i < (int)len; i += blockDim.x) {
+    const int pos = i % nap, q = i / nap;
+    const int zq = q % nz, ie = q / nz;
+    const int z = dz > 0 ? zq : nz - 1 - zq;
     const int ia = 16 * (pos >> 4) + PR::row((pos & 15) >> 2, pos & 3);
     const int e = xface ? sf_space_exp((oct & 1) ? p.nx : -1, c, z)
                         : sf_space_exp(c, (oct & 2) ? p.ny : -1, z);
-    dst[i] = ia < p.na ? (T)ldexp((double)((1 + ia) << (ia & 7)) * eu, e) : (T)0;
+    dst[i] = ia < p.na
+                 ? (T)ldexp((double)((1 + ia) << (ia & 7)) * (double)(sf_energy(ie) * su), e)
+                 : (T)0;
   }
 }
 
@@ -673,7 +729,7 @@ void launch_mfma_fill_multi(int precision, unsigned grid, const SweepParams* pro
     hipLaunchKernelGGL(k_fill_faces_multi<PrecF32>, dim3(grid), dim3(256), 0, st, probs, ml);
 }
 
-int mfma_fill_rows(const SweepParams& sp) { return kNoct * sp.ne * sp.nu * (sp.nx + sp.ny); }
+int mfma_fill_rows(const SweepParams& sp) { return kNoct * sp.nu * (sp.nx + sp.ny); }
 
 size_t mfma_face_elem_bytes(int precision) {
   return precision == MSW_PREC_FP64 ? sizeof(double) : sizeof(float);

@@ -57,6 +57,14 @@ int set_error(int code, const std::string& msg) {
 // per wave is fastest in FP64 (16 loses more to the cross-wave vo reduction
 // than it gains in occupancy) and 64 in the mixed forms; never more than na
 // rounded up to a 16-angle block, and at most 4 waves per tile.
+// Energy groups streamed through one MFMA tile pipeline: about 128 stream
+// steps per workgroup, so the 18-step fill and drain of a tile cost ~12%
+// rather than 18/(nz+18).
+int default_mfma_epw(int nz, int ne) {
+  const int e = (128 + nz / 2) / nz;
+  return e < 1 ? 1 : e > ne ? ne : e;
+}
+
 int default_mfma_nac(int precision, int na) {
   const int cap = na <= 16 ? 16 : na <= 32 ? 32 : 64;
   int nac = precision == MSW_PREC_FP64 ? 32 : 64;
@@ -507,6 +515,7 @@ struct MswPlan {
   int precision;
   int engine;  // MSW_ENGINE_*
   int nac, nwa, na_pad, ntx, nty;
+  int epw;  // energy groups per workgroup (MFMA engine)
   size_t lds_bytes;
   double* at;  // a_from_m transposed [8][na][nm] (+ padding)
   double* mf;  // m_from_a copy       [8][na][nm] (+ padding)
@@ -525,7 +534,8 @@ template <int NM, int NAC>
 static int launch_all(MswPlan* p, const SweepParams& sp, hipStream_t st) {
   const dim3 block(64 * p->nwa);
   for (size_t l = 0; l < p->launch_off.size(); ++l) {
-    const dim3 grid((unsigned)p->launch_cnt[l] * p->d.ne * p->d.nu);
+    const int nec = p->engine == MSW_ENGINE_MFMA ? (p->d.ne + p->epw - 1) / p->epw : p->d.ne;
+    const dim3 grid((unsigned)p->launch_cnt[l] * nec * p->d.nu);
     if (p->timing) HIP_TRY(hipEventRecord(p->ev_start[l], st));
     if (p->engine == MSW_ENGINE_MFMA) {
       LaunchSpec ls;
@@ -592,6 +602,7 @@ void msw_plan_default_options(MswPlanOptions* o) {
   o->precision = MSW_PREC_FP64;
   o->engine = MSW_ENGINE_MFMA;
   o->angles_per_wave = 0;
+  o->groups_per_workgroup = 0;
 }
 
 int msw_plan_create(const MswDims* d, int precision, MswPlan** out) {
@@ -647,6 +658,11 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
                    "", d->na);
   }
   p->na_pad = p->nwa * p->nac;
+  p->epw = 1;
+  if (opt->groups_per_workgroup < 0) {
+    delete p;
+    return set_err(MSW_ERR_INVALID, "groups_per_workgroup must be >= 0%s%ld");
+  }
   // tile of one wave: 8x8 cells (VALU engine), 16x4 (MFMA engine)
   const int tx = p->engine == MSW_ENGINE_MFMA ? mfma_tile_x() : kTile;
   const int ty = p->engine == MSW_ENGINE_MFMA ? mfma_tile_y() : kTile;
@@ -658,6 +674,21 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
     p->lds_bytes = sizeof(double) * ((size_t)p->nwa * 2 * kTile * (p->nac + 1) +
                                      (p->nwa > 1 ? (size_t)p->nwa * d->nm * 64 : 0));
   auto launches = build_schedule(p->ntx, p->nty, 0);
+  if (p->engine == MSW_ENGINE_MFMA) {
+    if (opt->groups_per_workgroup > 0) {
+      p->epw = std::min(opt->groups_per_workgroup, d->ne);
+    } else {
+      // stream groups only while the median launch keeps >= 4 waves per SIMD
+      // (1024 SIMDs) of one problem on its own
+      std::vector<int> cnt;
+      for (auto& l : launches) cnt.push_back((int)l.size());
+      std::sort(cnt.begin(), cnt.end());
+      const long long w1 = (long long)cnt[cnt.size() / 2] * d->nu * p->nwa;
+      int e = default_mfma_epw(d->ncell_z, d->ne);
+      while (e > 1 && w1 * ((d->ne + e - 1) / e) < 4096) --e;
+      p->epw = e;
+    }
+  }
   std::vector<TileItem> flat;
   for (auto& l : launches) {
     p->launch_off.push_back((int)flat.size());
@@ -774,6 +805,7 @@ int msw_plan_sweep(MswPlan* p, const double* vi, double* vo,
   sp.na_pad = p->na_pad;
   sp.ntx = p->ntx;
   sp.nty = p->nty;
+  sp.epw = p->epw;
   if (p->engine == MSW_ENGINE_MFMA) launch_mfma_fill(p->precision, sp, st);
   switch (d.nm) {
     case 4: return launch_nm<4>(p, sp, st);

@@ -119,3 +119,37 @@ def test_random_vs_oracle(shape, engine, cuda):
     n_ref, nd_ref = oracle.norms(od, vi0, vo_ref)
     assert abs(n - n_ref) <= NORM_TOL * n_ref
     assert abs(nd - nd_ref) <= NORM_TOL * nd_ref
+
+
+STREAM_SHAPES = [(9, 7, 3, 3, 16, 64, 4), (16, 4, 5, 4, 16, 32, 1), (5, 9, 1, 5, 4, 40, 2)]
+
+
+@pytest.mark.parametrize("shape", STREAM_SHAPES, ids=[str(s) for s in STREAM_SHAPES])
+def test_energy_streaming_vs_oracle(shape, cuda):
+    """Energy groups streamed through one tile pipeline (1, 2, 3 and all
+    groups per workgroup): FP64 within the parity bar of the oracle, and the
+    same bits for every grouping in FP64 and in the mixed FP32 form."""
+    import torch
+    print(f"case {shape}", flush=True)
+    d = msw.dims(*shape)
+    od = oracle.dims(*shape)
+    rng = np.random.default_rng(17)
+    base = oracle.init_state(od)
+    vi0 = base * (1.0 + 0.25 * rng.uniform(-1, 1, base.size))
+    a, m = oracle.init_matrices(od)
+    ref = oracle.sweep(od, a, m, vi0)
+    ta, tm = torch.from_numpy(a).to(cuda), torch.from_numpy(m).to(cuda)
+    vi = torch.from_numpy(vi0).to(cuda)
+    for prec in (msw.PREC_FP64, msw.PREC_MIXED_FP32):
+        outs = []
+        for epw in (1, 2, 3, shape[3]):
+            vo = torch.full_like(vi, float("nan"))
+            plan = msw.Plan(d, prec, engine=msw.ENGINE_MFMA, groups_per_workgroup=epw)
+            plan.sweep(vi, vo, ta, tm)
+            torch.cuda.synchronize()
+            plan.close()
+            outs.append(vo.cpu().numpy())
+        if prec == msw.PREC_FP64:
+            assert_close(outs[0], ref, what="vo vs oracle")
+        for o in outs[1:]:
+            np.testing.assert_array_equal(o, outs[0])
