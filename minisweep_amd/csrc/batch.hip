@@ -124,7 +124,7 @@ int make_geom(const MswDims& d, int nac, Geom* g) {
   g->nap = g->nwa * nac;
   g->ntx = (d.ncell_x + mfma_tile_x() - 1) / mfma_tile_x();
   g->nty = (d.ncell_y + mfma_tile_y() - 1) / mfma_tile_y();
-  g->epw = default_mfma_epw(d.ncell_z, d.ne);
+  g->epw = default_mfma_epw(d.ncell_z, d.ne);  // preferred; a tick may use fewer
   return MSW_OK;
 }
 
@@ -147,7 +147,6 @@ SweepParams make_params(const Geom& g, const double* vi, double* vo, const doubl
   sp.na_pad = g.nap;
   sp.ntx = g.ntx;
   sp.nty = g.nty;
-  sp.epw = g.epw;
   return sp;
 }
 
@@ -157,12 +156,31 @@ struct TickWork {
   const Geom* g;
   const DevSchedule* s;
   int launch;               // which launch of its schedule
+  int epw = 1;              // energy groups per workgroup in this tick
 };
+
+// Energy groups per workgroup for a tick: each problem's preferred stream
+// length, halved for all while the tick would have < 4 waves per SIMD.
+void choose_tick_epw(std::vector<TickWork>& work) {
+  for (TickWork& w : work) w.epw = w.g->epw;
+  for (;;) {
+    long long waves = 0;
+    int emax = 1;
+    for (const TickWork& w : work) {
+      waves += (long long)w.s->cnt[w.launch] * ((w.g->d.ne + w.epw - 1) / w.epw) * w.g->d.nu *
+               w.g->nwa;
+      emax = std::max(emax, w.epw);
+    }
+    if (waves >= 4096 || emax == 1) return;
+    for (TickWork& w : work) w.epw = std::max(1, w.epw / 2);
+  }
+}
 
 // Enqueue one combined tick: per kernel variant, k_sweep_mfma_multi launches
 // of up to kMaxSegments problems each.  Returns the number of launches.
-int enqueue_tick(int precision, const std::vector<TickWork>& work, const SweepParams* d_params,
+int enqueue_tick(int precision, std::vector<TickWork>& work, const SweepParams* d_params,
                  hipStream_t st) {
+  choose_tick_epw(work);
   std::map<std::tuple<int, int, int>, std::vector<const TickWork*>> by_variant;
   for (const TickWork& w : work) by_variant[w.g->variant()].push_back(&w);
   int nl = 0;
@@ -177,9 +195,9 @@ int enqueue_tick(int precision, const std::vector<TickWork>& work, const SweepPa
         ml.start[ml.nseg] = (int)grid;
         ml.prob[ml.nseg] = w.slot;
         ml.item_base[ml.nseg] = w.s->off[w.launch];
+        ml.epw[ml.nseg] = w.epw;
         ml.nseg++;
-        grid += (long long)w.s->cnt[w.launch] * ((w.g->d.ne + w.g->epw - 1) / w.g->epw) *
-                w.g->d.nu;
+        grid += (long long)w.s->cnt[w.launch] * ((w.g->d.ne + w.epw - 1) / w.epw) * w.g->d.nu;
       }
       const Geom& g = *ws[b]->g;
       launch_mfma_multi(precision, g.d.nm, g.nac, g.nwa, (unsigned)grid, d_params, ml, st);
@@ -202,6 +220,7 @@ int enqueue_fill(int precision, const std::vector<std::pair<int, const SweepPara
       ml.start[ml.nseg] = (int)grid;
       ml.prob[ml.nseg] = ps[k].first;
       ml.item_base[ml.nseg] = 0;
+      ml.epw[ml.nseg] = 1;
       ml.nseg++;
       grid += mfma_fill_rows(*ps[k].second);
     }

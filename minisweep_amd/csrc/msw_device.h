@@ -72,7 +72,6 @@ struct SweepParams {
   int nx, ny, nz, ne, na, nu;
   int na_pad;  // na rounded up to whole angle chunks (matrix/face stride)
   int ntx, nty;
-  int epw;     // energy groups streamed through one workgroup (MFMA kernels)
 };
 
 // A launch that runs segments of several sweeps: segment k covers
@@ -85,6 +84,7 @@ struct MultiLaunch {
   int start[kMaxSegments];
   int prob[kMaxSegments];
   int item_base[kMaxSegments];
+  int epw[kMaxSegments];  // energy groups streamed through one workgroup
 };
 
 }  // namespace msw

@@ -19,6 +19,7 @@ struct LaunchSpec {
   int nwa;           // waves per workgroup (angle chunks)
   int nac;           // angles per wave
   int nm;            // moments
+  int epw;           // energy groups streamed through one workgroup (MFMA)
   size_t lds_bytes;  // dynamic LDS per workgroup
 };
 
@@ -48,8 +49,11 @@ int mfma_tile_y();
 int set_error(int code, const std::string& msg);
 // Default angles per wave of the MFMA kernels for a precision and na.
 int default_mfma_nac(int precision, int na);
-// Default energy groups per workgroup of the MFMA kernels.
+// Energy groups per workgroup of the MFMA kernels: about 128 stream steps
+// per workgroup, then fewer while a launch of `waves1` waves at one group per
+// workgroup would drop below 4 waves per SIMD.
 int default_mfma_epw(int nz, int ne);
+int launch_mfma_epw(int preferred, int ne, long long waves1);
 
 // Host list schedule of (octant, tile) visits for ntx x nty tiles (kind 0:
 // most-remaining octant first; 1: the reference's octant order).

@@ -396,9 +396,12 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
 #pragma unroll
     for (int b = 0; b < kTY; ++b) vacc[b] = acc4{0, 0, 0, 0};
 
+    bool any_ent[kTY];
 #pragma unroll
-    for (int a = 0; a < NAB; ++a) {
-      acc4 vs[kTY];
+    for (int b = 0; b < kTY; ++b) any_ent[b] = __ballot(ent[b]) != 0;
+
+    // m2a of angle block a: VS[angle][cell] = AT . VI
+    auto m2a = [&](int a, acc4 (&vs)[kTY]) {
 #pragma unroll
       for (int b = kTY - 1; b >= 0; --b) {
         vs[b] = acc4{0, 0, 0, 0};
@@ -409,7 +412,18 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
           for (int s = 0; s < KM; ++s) vs[b] = PR::mma(aop[a][s], vin[b][s], vs[b]);
         }
       }
-      if (a == NAB - 1) fetch_vi(zn, en);
+    };
+    // The m2a of angle block a+1 is issued before the solve of block a, so the
+    // matrix cores work while the vector units solve.
+    acc4 vsc[kTY], vsn[kTY];
+    m2a(0, vsc);
+    if (NAB == 1) fetch_vi(zn, en);
+#pragma unroll
+    for (int a = 0; a < NAB; ++a) {
+      if (a + 1 < NAB) {
+        m2a(a + 1 < NAB ? a + 1 : 0, vsn);
+        if (a + 2 == NAB) fetch_vi(zn, en);
+      }
       // solve, blocks in descending order: block b's y face is block b-1's
       // value from the previous step, read before it is overwritten
 #pragma unroll
@@ -422,7 +436,7 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
         T rz[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) rz[r] = F[a][b][r];
-        if (__ballot(ent[b])) {  // some lane of this block starts a group
+        if (any_ent[b]) {  // some lane of this block starts a group
           const T sc = ent[b] ? (T)ldexp((double)(sf_energy(ie0 + es[b]) * su),
                                          sf_space_exp(x, yb[b], zb))
                               : (T)0;
@@ -434,21 +448,25 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
           const T rx = shr1_keep(xin[r], F[a][b][r]);
           const T ry = b > 0 ? F[a][b > 0 ? b - 1 : 0][r] : yinn[a][r];
           const T res = fmaT(rx, ldexpT(wz[r % PR::kWz], xe),
-                             fmaT(ry, cy, fmaT(rz[r], cz, (T)vs[b][r])));
+                             fmaT(ry, cy, fmaT(rz[r], cz, (T)vsc[b][r])));
           F[a][b][r] = res;
-          vs[b][r] = res;
+          vsc[b][r] = res;
         }
       }
       // a2m: VO[moment][cell] += M^T[moment][angle] R[angle][cell]
 #pragma unroll
       for (int b = 0; b < kTY; ++b) {
         if constexpr (PR::kPacked) {
-          T rv[4] = {(T)vs[b][0], (T)vs[b][1], (T)vs[b][2], (T)vs[b][3]};
+          T rv[4] = {(T)vsc[b][0], (T)vsc[b][1], (T)vsc[b][2], (T)vsc[b][3]};
           vacc[b] = PR::mma16(mop[a], rv, vacc[b]);
         } else {
 #pragma unroll
-          for (int s = 0; s < 4; ++s) vacc[b] = PR::mma(mop[a][s], (T)vs[b][s], vacc[b]);
+          for (int s = 0; s < 4; ++s) vacc[b] = PR::mma(mop[a][s], (T)vsc[b][s], vacc[b]);
         }
+      }
+      if (a + 1 < NAB) {
+#pragma unroll
+        for (int b = 0; b < kTY; ++b) vsc[b] = vsn[b];
       }
     }
     fetch_yin(t + 1);
@@ -516,23 +534,24 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
   }
 }
 
-// Workgroup bid of a launch -> (item, chunk of p.epw energy groups, unknown).
+// Workgroup bid of a launch -> (item, chunk of epw energy groups, unknown).
 template <class PR, int NM, int NAC, int NWA>
-__device__ __forceinline__ void sweep_block(const SweepParams& p, int item_base, int bid) {
+__device__ __forceinline__ void sweep_block(const SweepParams& p, int item_base, int bid,
+                                            int epw) {
   const int iu = bid % p.nu;
   bid /= p.nu;
-  const int nec = (p.ne + p.epw - 1) / p.epw;
+  const int nec = (p.ne + epw - 1) / epw;
   const int ec = bid % nec;
   bid /= nec;
-  const int ie0 = ec * p.epw;
-  sweep_tile<PR, NM, NAC, NWA>(p, p.items[item_base + bid], ie0, min(p.epw, p.ne - ie0), iu);
+  const int ie0 = ec * epw;
+  sweep_tile<PR, NM, NAC, NWA>(p, p.items[item_base + bid], ie0, min(epw, p.ne - ie0), iu);
 }
 
 // Single sweep.
 template <class PR, int NM, int NAC, int NWA>
 __global__ __launch_bounds__(64 * NWA) __attribute__((amdgpu_waves_per_eu(
-    Occupancy<PR, NAC>::value))) void k_sweep_mfma(SweepParams p, int item_base) {
-  sweep_block<PR, NM, NAC, NWA>(p, item_base, (int)blockIdx.x);
+    Occupancy<PR, NAC>::value))) void k_sweep_mfma(SweepParams p, int item_base, int epw) {
+  sweep_block<PR, NM, NAC, NWA>(p, item_base, (int)blockIdx.x, epw);
 }
 
 // Many sweeps in one launch: segment k of the launch is one launch of the
@@ -547,7 +566,7 @@ __global__ __launch_bounds__(64 * NWA) __attribute__((amdgpu_waves_per_eu(
   for (int step = kMaxSegments / 2; step > 0; step >>= 1)
     if (k + step < ml.nseg && ml.start[k + step] <= blk) k += step;
   const SweepParams p = probs[ml.prob[k]];
-  sweep_block<PR, NM, NAC, NWA>(p, ml.item_base[k], blk - ml.start[k]);
+  sweep_block<PR, NM, NAC, NWA>(p, ml.item_base[k], blk - ml.start[k], ml.epw[k]);
 }
 
 // Boundary inflow of every tile row and column: before the sweep, the x face
@@ -612,19 +631,19 @@ static void launch_nac(const LaunchSpec& ls, const SweepParams& sp, hipStream_t 
   switch (ls.nwa) {
     case 1:
       hipLaunchKernelGGL((k_sweep_mfma<PR, NM, NAC, 1>), dim3(ls.grid), block, ls.lds_bytes,
-                         st, sp, ls.item_base);
+                         st, sp, ls.item_base, ls.epw);
       break;
     case 2:
       hipLaunchKernelGGL((k_sweep_mfma<PR, NM, NAC, 2>), dim3(ls.grid), block, ls.lds_bytes,
-                         st, sp, ls.item_base);
+                         st, sp, ls.item_base, ls.epw);
       break;
     case 3:
       hipLaunchKernelGGL((k_sweep_mfma<PR, NM, NAC, 3>), dim3(ls.grid), block, ls.lds_bytes,
-                         st, sp, ls.item_base);
+                         st, sp, ls.item_base, ls.epw);
       break;
     default:
       hipLaunchKernelGGL((k_sweep_mfma<PR, NM, NAC, 4>), dim3(ls.grid), block, ls.lds_bytes,
-                         st, sp, ls.item_base);
+                         st, sp, ls.item_base, ls.epw);
       break;
   }
 }

@@ -65,6 +65,12 @@ int default_mfma_epw(int nz, int ne) {
   return e < 1 ? 1 : e > ne ? ne : e;
 }
 
+int launch_mfma_epw(int preferred, int ne, long long waves1) {
+  int e = preferred < 1 ? 1 : preferred > ne ? ne : preferred;
+  while (e > 1 && waves1 * ((ne + e - 1) / e) < 4096) --e;
+  return e;
+}
+
 int default_mfma_nac(int precision, int na) {
   const int cap = na <= 16 ? 16 : na <= 32 ? 32 : 64;
   int nac = precision == MSW_PREC_FP64 ? 32 : 64;
@@ -515,7 +521,7 @@ struct MswPlan {
   int precision;
   int engine;  // MSW_ENGINE_*
   int nac, nwa, na_pad, ntx, nty;
-  int epw;  // energy groups per workgroup (MFMA engine)
+  std::vector<int> launch_epw;  // energy groups per workgroup of each launch (MFMA)
   size_t lds_bytes;
   double* at;  // a_from_m transposed [8][na][nm] (+ padding)
   double* mf;  // m_from_a copy       [8][na][nm] (+ padding)
@@ -534,7 +540,8 @@ template <int NM, int NAC>
 static int launch_all(MswPlan* p, const SweepParams& sp, hipStream_t st) {
   const dim3 block(64 * p->nwa);
   for (size_t l = 0; l < p->launch_off.size(); ++l) {
-    const int nec = p->engine == MSW_ENGINE_MFMA ? (p->d.ne + p->epw - 1) / p->epw : p->d.ne;
+    const int epw = p->engine == MSW_ENGINE_MFMA ? p->launch_epw[l] : 1;
+    const int nec = (p->d.ne + epw - 1) / epw;
     const dim3 grid((unsigned)p->launch_cnt[l] * nec * p->d.nu);
     if (p->timing) HIP_TRY(hipEventRecord(p->ev_start[l], st));
     if (p->engine == MSW_ENGINE_MFMA) {
@@ -545,6 +552,7 @@ static int launch_all(MswPlan* p, const SweepParams& sp, hipStream_t st) {
       ls.nac = p->nac;
       ls.nm = p->d.nm;
       ls.lds_bytes = p->lds_bytes;
+      ls.epw = epw;
       launch_mfma(p->precision, ls, sp, st);
     } else {
       hipLaunchKernelGGL((k_sweep_fp64<NM, NAC>), grid, block, p->lds_bytes, st, sp,
@@ -658,7 +666,6 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
                    "", d->na);
   }
   p->na_pad = p->nwa * p->nac;
-  p->epw = 1;
   if (opt->groups_per_workgroup < 0) {
     delete p;
     return set_err(MSW_ERR_INVALID, "groups_per_workgroup must be >= 0%s%ld");
@@ -674,20 +681,13 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
     p->lds_bytes = sizeof(double) * ((size_t)p->nwa * 2 * kTile * (p->nac + 1) +
                                      (p->nwa > 1 ? (size_t)p->nwa * d->nm * 64 : 0));
   auto launches = build_schedule(p->ntx, p->nty, 0);
-  if (p->engine == MSW_ENGINE_MFMA) {
-    if (opt->groups_per_workgroup > 0) {
-      p->epw = std::min(opt->groups_per_workgroup, d->ne);
-    } else {
-      // stream groups only while the median launch keeps >= 4 waves per SIMD
-      // (1024 SIMDs) of one problem on its own
-      std::vector<int> cnt;
-      for (auto& l : launches) cnt.push_back((int)l.size());
-      std::sort(cnt.begin(), cnt.end());
-      const long long w1 = (long long)cnt[cnt.size() / 2] * d->nu * p->nwa;
-      int e = default_mfma_epw(d->ncell_z, d->ne);
-      while (e > 1 && w1 * ((d->ne + e - 1) / e) < 4096) --e;
-      p->epw = e;
-    }
+  // energy groups per workgroup, per launch: streamed while the launch
+  // keeps >= 4 waves per SIMD (1024 SIMDs), or as the caller fixed it
+  for (auto& l : launches) {
+    const long long w1 = (long long)l.size() * d->nu * p->nwa;
+    p->launch_epw.push_back(opt->groups_per_workgroup > 0
+                                ? std::min(opt->groups_per_workgroup, d->ne)
+                                : launch_mfma_epw(default_mfma_epw(d->ncell_z, d->ne), d->ne, w1));
   }
   std::vector<TileItem> flat;
   for (auto& l : launches) {
@@ -805,7 +805,6 @@ int msw_plan_sweep(MswPlan* p, const double* vi, double* vo,
   sp.na_pad = p->na_pad;
   sp.ntx = p->ntx;
   sp.nty = p->nty;
-  sp.epw = p->epw;
   if (p->engine == MSW_ENGINE_MFMA) launch_mfma_fill(p->precision, sp, st);
   switch (d.nm) {
     case 4: return launch_nm<4>(p, sp, st);
