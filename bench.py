@@ -96,7 +96,7 @@ def time_precision(msw, torch, d, prec_name, args, ta, tm, vi, vo, flops_sweep):
             "bf16": msw.PREC_MIXED_BF16}[prec_name]
     engine = msw.ENGINE_MFMA if args.engine == "mfma" else msw.ENGINE_VALU
     plan = msw.Plan(d, prec, engine=engine, angles_per_wave=args.angles_per_wave,
-                    groups_per_workgroup=args.groups_per_workgroup)
+                    groups_per_workgroup=args.groups_per_workgroup, tile_rows=args.tile_rows)
     stream = torch.cuda.current_stream()
     bufs = [vi, vo]
     for i in range(args.warmup):
@@ -143,6 +143,7 @@ def main():
                          "mixed forms)")
     ap.add_argument("--angles-per-wave", type=int, default=0)
     ap.add_argument("--groups-per-workgroup", type=int, default=0)
+    ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="threads for the CPU baseline (0: OMP_NUM_THREADS or 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")

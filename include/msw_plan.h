@@ -70,7 +70,8 @@ typedef struct {
   int engine;          /* MSW_ENGINE_* */
   int angles_per_wave; /* 0: chosen from na; MFMA: 16, 32 or 64; VALU: 8, 16 or 32 */
   int groups_per_workgroup; /* MFMA: energy groups streamed through one tile
-                               pipeline (0: chosen from ncell_z) */
+                               pipeline (0: chosen per launch from ncell_z) */
+  int tile_rows;            /* MFMA: cell rows per tile, 4, 8 or 16 (0: chosen) */
 } MswPlanOptions;
 
 void msw_plan_default_options(MswPlanOptions* opt);

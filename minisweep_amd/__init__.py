@@ -38,7 +38,8 @@ class MswDims(ctypes.Structure):
 
 class MswPlanOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("engine", ctypes.c_int),
-                ("angles_per_wave", ctypes.c_int), ("groups_per_workgroup", ctypes.c_int)]
+                ("angles_per_wave", ctypes.c_int), ("groups_per_workgroup", ctypes.c_int),
+                ("tile_rows", ctypes.c_int)]
 
 
 class MswBatchOptions(ctypes.Structure):
@@ -172,7 +173,7 @@ class Plan:
 
     def __init__(self, d: MswDims, precision: int = PREC_FP64,
                  engine: Optional[int] = None, angles_per_wave: int = 0,
-                 groups_per_workgroup: int = 0):
+                 groups_per_workgroup: int = 0, tile_rows: int = 0):
         self.d = d
         self.precision = precision
         o = MswPlanOptions()
@@ -182,6 +183,7 @@ class Plan:
             o.engine = engine
         o.angles_per_wave = angles_per_wave
         o.groups_per_workgroup = groups_per_workgroup
+        o.tile_rows = tile_rows
         self.engine = o.engine
         h = ctypes.c_void_p()
         _check(lib().msw_plan_create_ex(ctypes.byref(d), ctypes.byref(o), ctypes.byref(h)),

@@ -92,7 +92,7 @@ void free_schedules(ScheduleCache& c) {
 // Geometry of one problem under the MFMA kernels.
 struct Geom {
   MswDims d{};
-  int nac = 16, nwa = 1, nap = 16, ntx = 1, nty = 1, epw = 1;
+  int nac = 16, nwa = 1, nap = 16, ntx = 1, nty = 1, epw = 1, ny = 1;
   size_t mat_doubles() const { return ((size_t)kNoct * nap + nac) * d.nm; }
   size_t face_elems() const {
     return (size_t)kNoct * d.ne * d.nu * (d.ncell_x + d.ncell_y) * d.ncell_z * nap;
@@ -100,7 +100,7 @@ struct Geom {
   size_t state_doubles() const {
     return (size_t)d.ncell_x * d.ncell_y * d.ncell_z * d.ne * d.nm * d.nu;
   }
-  std::tuple<int, int, int> variant() const { return std::make_tuple(d.nm, nac, nwa); }
+  std::tuple<int, int, int, int> variant() const { return std::make_tuple(d.nm, nac, nwa, ny); }
 };
 
 int check_dims_mfma(const MswDims& d) {
@@ -112,7 +112,7 @@ int check_dims_mfma(const MswDims& d) {
   return MSW_OK;
 }
 
-int make_geom(const MswDims& d, int nac, Geom* g) {
+int make_geom(const MswDims& d, int nac, Geom* g, int ny = 0, int precision = MSW_PREC_FP64) {
   int rc = check_dims_mfma(d);
   if (rc) return rc;
   if (nac != 16 && nac != 32 && nac != 64)
@@ -123,14 +123,25 @@ int make_geom(const MswDims& d, int nac, Geom* g) {
   if (g->nwa > 4) return set_error(MSW_ERR_UNSUPPORTED, "na above 4 waves of angles");
   g->nap = g->nwa * nac;
   g->ntx = (d.ncell_x + mfma_tile_x() - 1) / mfma_tile_x();
-  g->nty = (d.ncell_y + mfma_tile_y() - 1) / mfma_tile_y();
+  // requested row groups, or fewer where that workgroup shape is not built
+  g->ny = ny > 0 ? ny : default_mfma_ny(precision, g->nwa, d.ncell_y);
+  while (g->ny > 1 && !mfma_shape_supported(g->nwa, g->ny)) g->ny /= 2;
+  const int rows = mfma_tile_rows() * g->ny;
+  g->nty = (d.ncell_y + rows - 1) / rows;
   g->epw = default_mfma_epw(d.ncell_z, d.ne);  // preferred; a tick may use fewer
   return MSW_OK;
 }
 
+// FP32 running vo sum of the mixed forms (none in FP64).
+size_t vo32_bytes(const Geom& g, int precision) {
+  return precision == MSW_PREC_FP64 ? 0 : align_up(sizeof(float) * g.state_doubles());
+}
+
 SweepParams make_params(const Geom& g, const double* vi, double* vo, const double* at,
-                        const double* mf, void* fyz, void* fxz, const DevSchedule* s) {
+                        const double* mf, void* fyz, void* fxz, float* vo32,
+                        const DevSchedule* s) {
   SweepParams sp;
+  sp.vo32 = vo32;
   sp.vi = vi;
   sp.vo = vo;
   sp.at = at;
@@ -181,7 +192,7 @@ void choose_tick_epw(std::vector<TickWork>& work) {
 int enqueue_tick(int precision, std::vector<TickWork>& work, const SweepParams* d_params,
                  hipStream_t st) {
   choose_tick_epw(work);
-  std::map<std::tuple<int, int, int>, std::vector<const TickWork*>> by_variant;
+  std::map<std::tuple<int, int, int, int>, std::vector<const TickWork*>> by_variant;
   for (const TickWork& w : work) by_variant[w.g->variant()].push_back(&w);
   int nl = 0;
   for (auto& kv : by_variant) {
@@ -200,7 +211,7 @@ int enqueue_tick(int precision, std::vector<TickWork>& work, const SweepParams* 
         grid += (long long)w.s->cnt[w.launch] * ((w.g->d.ne + w.epw - 1) / w.epw) * w.g->d.nu;
       }
       const Geom& g = *ws[b]->g;
-      launch_mfma_multi(precision, g.d.nm, g.nac, g.nwa, (unsigned)grid, d_params, ml, st);
+      launch_mfma_multi(precision, g.d.nm, g.nac, g.nwa, g.ny, (unsigned)grid, d_params, ml, st);
       ++nl;
     }
   }
@@ -253,6 +264,7 @@ struct BatchProb {
   double* mf = nullptr;
   void* fyz = nullptr;
   void* fxz = nullptr;
+  float* vo32 = nullptr;
   int start_tick = 0;
 };
 
@@ -273,6 +285,7 @@ struct MswBatch {
   hipGraphExec_t exec = nullptr;
   int nkernels = 0, nticks = 0;
   int stream_target = 128;  // z-stream steps per workgroup (energy groups streamed)
+  int row_groups = 0;       // row groups of 4 rows per tile (0: the plan default)
   std::string log;
 };
 
@@ -343,7 +356,7 @@ int batch_enqueue(MswBatch* b, hipStream_t st) {
   std::vector<std::pair<int, const SweepParams*>> fill;
   for (size_t i = 0; i < b->probs.size(); ++i) {
     const BatchProb& p = b->probs[i];
-    hp[i] = make_params(p.g, p.vi, p.vo, p.at, p.mf, p.fyz, p.fxz, p.s);
+    hp[i] = make_params(p.g, p.vi, p.vo, p.at, p.mf, p.fyz, p.fxz, p.vo32, p.s);
   }
   for (size_t i = 0; i < b->probs.size(); ++i) fill.emplace_back((int)i, &hp[i]);
   nk += enqueue_fill(prec, fill, b->d_params, st);
@@ -374,7 +387,7 @@ int batch_build(MswBatch* b) {
   std::vector<size_t> offs(b->probs.size());
   for (size_t i = 0; i < b->probs.size(); ++i) {
     BatchProb& p = b->probs[i];
-    int rc = make_geom(b->dims[i], nac_for(b, b->dims[i]), &p.g);
+    int rc = make_geom(b->dims[i], nac_for(b, b->dims[i]), &p.g, b->row_groups, b->opt.precision);
     if (rc) return rc;
     const int nz = p.g.d.ncell_z;
     p.g.epw = std::max(1, std::min(p.g.d.ne, (b->stream_target + nz / 2) / nz));
@@ -384,7 +397,7 @@ int batch_build(MswBatch* b) {
     const size_t nfx = (size_t)kNoct * p.g.d.ne * p.g.d.nu * p.g.d.ncell_y * p.g.d.ncell_z * p.g.nap;
     const size_t nfy = (size_t)kNoct * p.g.d.ne * p.g.d.nu * p.g.d.ncell_x * p.g.d.ncell_z * p.g.nap;
     total += 2 * align_up(sizeof(double) * p.g.mat_doubles()) + align_up(fsz * nfx) +
-             align_up(fsz * nfy);
+             align_up(fsz * nfy) + vo32_bytes(p.g, b->opt.precision);
   }
   HIP_OK(hipMalloc(&b->arena, total), "batch arena");
   b->arena_bytes = total;
@@ -397,15 +410,19 @@ int batch_build(MswBatch* b) {
     p.mf = (double*)(q + mb);
     q += 2 * mb;
     const size_t nfx = (size_t)kNoct * p.g.d.ne * p.g.d.nu * p.g.d.ncell_y * p.g.d.ncell_z * p.g.nap;
+    const size_t nfy = (size_t)kNoct * p.g.d.ne * p.g.d.nu * p.g.d.ncell_x * p.g.d.ncell_z * p.g.nap;
     p.fyz = q;
     p.fxz = q + align_up(fsz * nfx);
+    p.vo32 = vo32_bytes(p.g, b->opt.precision)
+                 ? (float*)(q + align_up(fsz * nfx) + align_up(fsz * nfy))
+                 : nullptr;
     p.start_tick = 0;
   }
   if (b->opt.stagger) stagger_starts(b);
   std::vector<SweepParams> hp(b->probs.size());
   for (size_t i = 0; i < b->probs.size(); ++i) {
     const BatchProb& p = b->probs[i];
-    hp[i] = make_params(p.g, p.vi, p.vo, p.at, p.mf, p.fyz, p.fxz, p.s);
+    hp[i] = make_params(p.g, p.vi, p.vo, p.at, p.mf, p.fyz, p.fxz, p.vo32, p.s);
   }
   HIP_OK(hipMalloc(&b->d_params, sizeof(SweepParams) * hp.size()), "batch params");
   HIP_OK(hipMemcpy(b->d_params, hp.data(), sizeof(SweepParams) * hp.size(), hipMemcpyHostToDevice),
@@ -564,6 +581,21 @@ int msw_batch_autotune(MswBatch* b, void* stream, int reps) {
     b->class_nac[c] = keep;
     b->built = false;
   }
+  for (int rg : {1, 2, 4}) {
+    if (rg == b->row_groups) continue;
+    const int save = b->row_groups;
+    b->row_groups = rg;
+    b->built = false;
+    double ms = 0;
+    rc = batch_time(b, st, reps, &ms);
+    if (rc) return rc;
+    note("row_groups" + std::to_string(rg), ms);
+    if (ms < best)
+      best = ms;
+    else
+      b->row_groups = save;
+    b->built = false;
+  }
   for (int target : {64, 256}) {
     const int save = b->stream_target;
     b->stream_target = target;
@@ -611,6 +643,7 @@ int msw_batch_describe(const MswBatch* b, char* buf, int len) {
                   ", \"stagger\": " + std::to_string(b->opt.stagger) +
                   ", \"graph\": " + std::to_string(b->opt.use_graph) +
                   ", \"stream_target\": " + std::to_string(b->stream_target) +
+                  ", \"row_groups\": " + std::to_string(b->row_groups) +
                   ", \"ticks\": " + std::to_string(b->nticks) +
                   ", \"kernels_per_run\": " + std::to_string(b->nkernels) + ", \"class_nac\": {";
   bool first = true;
@@ -709,6 +742,7 @@ struct Active {
   double* part = nullptr;
   void* fyz = nullptr;
   void* fxz = nullptr;
+  float* vo32 = nullptr;
   const double* at = nullptr;
   const double* mf = nullptr;
   int next = 0;  // next launch of its schedule
@@ -790,7 +824,8 @@ size_t footprint(const Geom& g, int precision) {
   const size_t nfx = (size_t)kNoct * g.d.ne * g.d.nu * g.d.ncell_y * g.d.ncell_z * g.nap;
   const size_t nfy = (size_t)kNoct * g.d.ne * g.d.nu * g.d.ncell_x * g.d.ncell_z * g.nap;
   return 2 * align_up(sizeof(double) * g.state_doubles()) + align_up(fsz * nfx) +
-         align_up(fsz * nfy) + align_up(sizeof(double) * norms_scratch_doubles());
+         align_up(fsz * nfy) + align_up(sizeof(double) * norms_scratch_doubles()) +
+         vo32_bytes(g, precision);
 }
 
 void finish(MswServer* s, Active& a, int status) {
@@ -831,7 +866,8 @@ void server_loop(MswServer* s) {
         Request& q = s->queue.front();
         Active a;
         a.req = q;
-        int rc = make_geom(q.d, default_mfma_nac(s->opt.precision, q.d.na), &a.g);
+        int rc = make_geom(q.d, default_mfma_nac(s->opt.precision, q.d.na), &a.g, 0,
+                           s->opt.precision);
         if (!rc) rc = get_schedule(s->scheds, a.g.ntx, a.g.nty, 0, &a.s);
         if (!rc) rc = server_matrices(s, a.g, &a.at, &a.mf);
         if (rc) {
@@ -869,6 +905,9 @@ void server_loop(MswServer* s) {
         a.fyz = q0 + 2 * sb;
         a.fxz = q0 + 2 * sb + align_up(fsz * nfx);
         a.part = (double*)(q0 + 2 * sb + align_up(fsz * nfx) + align_up(fsz * nfy));
+        a.vo32 = vo32_bytes(a.g, s->opt.precision)
+                     ? (float*)((char*)a.part + align_up(sizeof(double) * norms_scratch_doubles()))
+                     : nullptr;
         a.t_admit = now_s();
         s->active.push_back(a);
         fresh.push_back(s->active.size() - 1);
@@ -885,7 +924,8 @@ void server_loop(MswServer* s) {
       std::vector<std::pair<int, const SweepParams*>> fill;
       for (size_t k : fresh) {
         Active& a = s->active[k];
-        s->h_params[a.slot] = make_params(a.g, a.vi, a.vo, a.at, a.mf, a.fyz, a.fxz, a.s);
+        s->h_params[a.slot] =
+            make_params(a.g, a.vi, a.vo, a.at, a.mf, a.fyz, a.fxz, a.vo32, a.s);
         (void)hipMemcpyAsync(s->d_params + a.slot, s->h_params + a.slot, sizeof(SweepParams),
                              hipMemcpyHostToDevice, s->st);
         if (a.req.vi_pinned)

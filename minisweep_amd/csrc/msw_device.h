@@ -15,11 +15,15 @@ constexpr int kTile = 8;      // xy tile edge: one wave64 = 8x8 cells
 constexpr int kSkew = 2 * (kTile - 1);
 
 // One (octant, tile) visit in a launch.  Tile coordinates are in the
-// octant's own frame (upwind corner at 0,0).  first != 0: this visit is the
-// first of the sweep to reach the tile, so it writes vo instead of adding.
+// octant's own frame (upwind corner at 0,0).  flags: kVisitFirst, this visit
+// is the first of the sweep to reach the tile, so it writes vo instead of
+// adding; kVisitLast, no later visit reaches it (the mixed forms then write
+// the FP64 vo from their FP32 running sum).
+constexpr int kVisitFirst = 1;
+constexpr int kVisitLast = 2;
 struct TileItem {
   int octant;
-  int first;
+  int flags;
   int tx;
   int ty;
 };
@@ -68,6 +72,7 @@ struct SweepParams {
   void* fyz;         // x-direction faces: VALU [8][ne][nu][ny][nz][na] (grid frame),
                      //   MFMA [8][nu][ny][ne][nz][na] (octant-frame z)
   void* fxz;         // y-direction faces: the same with nx for ny
+  float* vo32;       // mixed forms: FP32 running sum of vo over the octants (state layout)
   const TileItem* items;
   int nx, ny, nz, ne, na, nu;
   int na_pad;  // na rounded up to whole angle chunks (matrix/face stride)

@@ -20,20 +20,23 @@ struct LaunchSpec {
   int nac;           // angles per wave
   int nm;            // moments
   int epw;           // energy groups streamed through one workgroup (MFMA)
+  int ny;            // row groups of 4 rows per tile (MFMA)
   size_t lds_bytes;  // dynamic LDS per workgroup
 };
 
 // Sweep with m2a/a2m on MFMA (sweep_mfma.hip), in the plan's precision
-// (MSW_PREC_*).  nac is 16, 32 or 64; nwa 1..4; nm 4, 8 or 16.
+// (MSW_PREC_*).  nac is 16, 32 or 64; nm 4, 8 or 16; (nwa, ny) a shape for
+// which mfma_shape_supported is true.
 void launch_mfma(int precision, const LaunchSpec& ls, const SweepParams& sp, hipStream_t st);
-size_t mfma_lds_bytes(int precision, int nac, int nwa);
+bool mfma_shape_supported(int nwa, int ny);
+size_t mfma_lds_bytes(int precision, int nac, int nwa, int ny);
 // Writes the domain-boundary inflow into the MFMA kernels' face planes; runs
 // before the launches of every sweep.
 void launch_mfma_fill(int precision, const SweepParams& sp, hipStream_t st);
 // Several sweeps' launches in one kernel launch (one kernel variant: all
 // segments' problems share nm, nac, nwa and the precision).  probs is a
 // device array of the problems' parameters.
-void launch_mfma_multi(int precision, int nm, int nac, int nwa, unsigned grid,
+void launch_mfma_multi(int precision, int nm, int nac, int nwa, int ny, unsigned grid,
                        const SweepParams* probs, const MultiLaunch& ml, hipStream_t st);
 void launch_mfma_fill_multi(int precision, unsigned grid, const SweepParams* probs,
                             const MultiLaunch& ml, hipStream_t st);
@@ -41,9 +44,10 @@ void launch_mfma_fill_multi(int precision, unsigned grid, const SweepParams* pro
 int mfma_fill_rows(const SweepParams& sp);
 // Bytes of one face value of the MFMA kernels in a precision.
 size_t mfma_face_elem_bytes(int precision);
-// Tile of one wave of the MFMA kernels (cells along x and y).
+// MFMA tile: mfma_tile_x() cells along x, ny row groups of mfma_tile_rows()
+// rows along y.
 int mfma_tile_x();
-int mfma_tile_y();
+int mfma_tile_rows();
 
 // Sets the calling thread's msw_last_error() text; returns code.
 int set_error(int code, const std::string& msg);
@@ -54,6 +58,8 @@ int default_mfma_nac(int precision, int na);
 // workgroup would drop below 4 waves per SIMD.
 int default_mfma_epw(int nz, int ne);
 int launch_mfma_epw(int preferred, int ne, long long waves1);
+// Default row groups per MFMA tile.
+int default_mfma_ny(int precision, int nwa, int ncell_y);
 
 // Host list schedule of (octant, tile) visits for ntx x nty tiles (kind 0:
 // most-remaining octant first; 1: the reference's octant order).

@@ -1,6 +1,7 @@
 // sweep_mfma.hip -- the sweep with m2a/a2m on the gfx950 matrix cores, in
 // FP64 (plain) and in mixed precision (FP32 or BF16 contractions with FP32
-// accumulation, FP32 faces and solve, FP64 state and vo accumulation).
+// accumulation, FP32 faces and solve, FP64 input state; vo summed over the
+// octants in FP32 and written in FP64 by each tile's last octant visit).
 //
 // One wave owns a tile of 16 (x) by 4 (y) cells for one (octant, energy
 // group, unknown, chunk of NAC angles) and walks the whole z extent.  Lane
@@ -49,6 +50,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "msw_device.h"
 #include "msw_internal.h"
@@ -177,7 +179,7 @@ struct Occupancy {
 // angle chunk = wave index.  Lane (j, b) is at stream position
 // t - j - b = (group - ie0) * nz + z, so the tile's fill and drain are paid
 // once per nge groups.
-template <class PR, int NM, int NAC, int NWA>
+template <class PR, int NM, int NAC, int NWA, int NY>
 __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem it,
                                            const int ie0, const int nge, const int iu) {
   typedef typename PR::T T;
@@ -188,10 +190,14 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
   constexpr int KA = PR::kPacked ? 4 : KM;     // m2a operand values per lane and block
   constexpr int MSTEP = PR::kPacked ? 4 : KM;  // first m2a moment of lane group g: MSTEP*g
   constexpr int NOWN = (kTY + NWA - 1) / NWA;  // cell blocks whose vo one wave stores
+  constexpr bool kF64 = std::is_same<typename PR::T, double>::value;
   static_assert(NAC % 16 == 0 && NM % 4 == 0 && NM <= 16, "block shapes");
 
+  constexpr int kTYT = kTY * NY;                // tile rows: NY waves of 4 rows
   const int lane = threadIdx.x & 63;
-  const int wave = NWA == 1 ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wave = NWA * NY == 1 ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wa = NWA == 1 ? 0 : wave % NWA;     // angle chunk of this wave
+  const int wyi = NY == 1 ? 0 : wave / NWA;     // row group of this wave: rows 4 wyi..4 wyi+3
   const int j = lane & 15, g = lane >> 4;
   const int oct = it.octant;
   const int dx = (oct & 1) ? -1 : 1;
@@ -200,16 +206,19 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
   const int nx = p.nx, ny = p.ny, nz = p.nz, nu = p.nu, nap = p.na_pad;
   const int gx = dx > 0 ? it.tx : p.ntx - 1 - it.tx;
   const int gy = dy > 0 ? it.ty : p.nty - 1 - it.ty;
-  const int ia0 = wave * NAC;
-  const bool first = it.first != 0;
+  const int ia0 = wa * NAC;
+  const bool first = (it.flags & kVisitFirst) != 0;
+  const bool last = (it.flags & kVisitLast) != 0;
   const bool x_out = it.tx + 1 < p.ntx, y_out = it.ty + 1 < p.nty;
 
   // ---- the lane's cells ------------------------------------------------------
-  // The tile covers grid cells [16 gx, 16 gx + wx) x [4 gy, 4 gy + wy) for
-  // every octant; lane j = 0 and block b = 0 are its upwind-most column and
-  // row in the octant's direction, so a partial tile's missing cells are
-  // always downstream (lanes j >= wx, blocks b >= wy).
-  const int wx = min(kTX, nx - gx * kTX), wy = min(kTY, ny - gy * kTY);
+  // The tile covers grid cells [16 gx, 16 gx + wx) x [kTYT gy, kTYT gy + wy)
+  // for every octant; lane j = 0 and row 0 are its upwind-most column and row
+  // in the octant's direction, so a partial tile's missing cells are always
+  // downstream (lanes j >= wx, rows >= wy).  Wave row group wyi owns rows
+  // 4 wyi + b and runs 4 wyi steps behind row group 0, so that its row 0
+  // meets the row above it one step after that row was solved.
+  const int wx = min(kTX, nx - gx * kTX), wy = min(kTYT, ny - gy * kTYT);
   const bool x_ok = j < wx;
   const int jc = x_ok ? j : wx - 1;  // clamped: loads never leave the arrays
   const int x = dx > 0 ? gx * kTX + jc : gx * kTX + wx - 1 - jc;
@@ -222,17 +231,20 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
   size_t coff[kTY];
 #pragma unroll
   for (int b = 0; b < kTY; ++b) {
-    const int bc = b < wy ? b : wy - 1;
-    yb[b] = dy > 0 ? gy * kTY + bc : gy * kTY + wy - 1 - bc;
+    const int r = kTY * wyi + b;
+    const int bc = r < wy ? r : wy - 1;
+    yb[b] = dy > 0 ? gy * kTYT + bc : gy * kTYT + wy - 1 - bc;
     coff[b] = cxo + (size_t)yb[b] * nx * nu * NM;
     cst[b] = x + 3 * yb[b] + 2;
   }
   const int zb = dz > 0 ? -1 : nz;
 
-  // ---- LDS: vo partial sums [2][kTY][NWA][64][4], x inflow staging [NWA][NAB][64]
+  // ---- LDS: vo partial sums [2][NY][kTY][NWA][64][4], y faces between row
+  // groups [2][NY-1][NWA][NAB][64][4], x inflow staging [NY*NWA][NAB][64] --
   extern __shared__ double lds_raw[];
   A* red = reinterpret_cast<A*>(lds_raw);
-  T* xs_lds = reinterpret_cast<T*>(red + (NWA > 1 ? 2 * kTY * NWA * 64 * 4 : 0)) +
+  T* yring = reinterpret_cast<T*>(red + (NWA > 1 ? 2 * NY * kTY * NWA * 64 * 4 : 0));
+  T* xs_lds = yring + (NY > 1 ? 2 * (NY - 1) * NWA * NAB * 64 * 4 : 0) +
               (size_t)wave * NAB * 64;
 
   // ---- face planes of this (octant, unknown): [y or x][group][z][angle],
@@ -287,11 +299,14 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
   // path, the last step's too (z is clamped), so that the memory counters
   // agree across paths and each wait covers only the data it needs. ---------
   T vin[kTY][KA];        // m2a B operand
-  double von[NOWN][4];   // old vo of the owned blocks' cells
+  // running vo sum of the owned blocks' cells from earlier octant visits:
+  // FP64 vo in the FP64 form, the FP32 sum in the mixed forms
+  typedef typename std::conditional<kF64, double, float>::type V;
+  V von[NOWN][4];
   T xsn[NAB];            // x inflow, spread: lane (j, g) holds block j/4, register j%4
   T yinn[NAB][4];        // y inflow of block 0
   const int ns = nge * nz;  // stream length of this workgroup
-  const int nsteps = ns + kTX + kTY - 2;
+  const int nsteps = ns + kTX + kTYT - 2;
   const bool mpad = moff + KA > NM;  // packed operand past the last moment
 
   // Stream position of block b's cell: (group offset es, z in the octant's
@@ -300,7 +315,7 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
   int zs[kTY], es[kTY];
 #pragma unroll
   for (int b = 0; b < kTY; ++b) {
-    zs[b] = -j - b;
+    zs[b] = -j - b - kTY * wyi;
     es[b] = 0;
   }
   // state offset of (group offset e, octant-frame z) in a block, clamped
@@ -342,18 +357,20 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
   auto fetch_vo = [&](const int (&ze)[kTY], const int (&ee)[kTY]) {
 #pragma unroll
     for (int b = 0; b < kTY; ++b)
-      if (b % NWA == wave && 4 * g < NM)
-        load4(p.vo + soff(b, ee[b], ze[b]) + 4 * g, von[b / NWA]);
+      if (b % NWA == wa && 4 * g < NM)
+        load4((kF64 ? (const V*)p.vo : (const V*)p.vo32) + soff(b, ee[b], ze[b]) + 4 * g,
+              von[b / NWA]);
   };
   auto fetch_xs = [&](int t) {
     const int bb = j >> 2;
-    const int bc = bb < wy ? bb : wy - 1;
-    const int y = dy > 0 ? gy * kTY + bc : gy * kTY + wy - 1 - bc;
-    const T* src = fyz + (size_t)y * nze * nap + foff(t - bb) + (j & 3);
+    const int r = kTY * wyi + bb;
+    const int bc = r < wy ? r : wy - 1;
+    const int y = dy > 0 ? gy * kTYT + bc : gy * kTYT + wy - 1 - bc;
+    const T* src = fyz + (size_t)y * nze * nap + foff(t - bb - kTY * wyi) + (j & 3);
 #pragma unroll
     for (int a = 0; a < NAB; ++a) xsn[a] = src[16 * a];
   };
-  auto fetch_yin = [&](int t) {
+  auto fetch_yin = [&](int t) {  // row group 0 only
     const T* src = fxz + (size_t)x * nze * nap + foff(t - j);
 #pragma unroll
     for (int a = 0; a < NAB; ++a) load4(src + 16 * a, yinn[a]);
@@ -361,7 +378,7 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
 
   fetch_xs(0);
   fetch_vi(zs, es);
-  fetch_yin(0);
+  if (wyi == 0) fetch_yin(0);
   fetch_vo(zs, es);
 
   for (int t = 0; t < nsteps; ++t) {
@@ -384,7 +401,7 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
     size_t so[kTY];
 #pragma unroll
     for (int b = 0; b < kTY; ++b) {
-      val[b] = x_ok && b < wy && zs[b] >= 0 && es[b] < nge;
+      val[b] = x_ok && kTY * wyi + b < wy && zs[b] >= 0 && es[b] < nge;
       ent[b] = zs[b] == 0 && es[b] < nge;
       const int zc = zs[b] < 0 ? 0 : zs[b];
       zgs[b] = dz > 0 ? zc : nz - 1 - zc;
@@ -426,6 +443,13 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
       }
       // solve, blocks in descending order: block b's y face is block b-1's
       // value from the previous step, read before it is overwritten
+      // y inflow of row 0 of a lower row group: the row group above wrote it
+      // into the ring in the previous step
+      T yr[4];
+      if (NY > 1 && wyi > 0)
+        load4(yring + ((((size_t)((t + 1) & 1) * (NY - 1) + (wyi - 1)) * NWA + wa) * NAB + a) *
+                          256 + lane * 4,
+              yr);
 #pragma unroll
       for (int b = kTY - 1; b >= 0; --b) {
         T xin[4];
@@ -446,7 +470,8 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const T rx = shr1_keep(xin[r], F[a][b][r]);
-          const T ry = b > 0 ? F[a][b > 0 ? b - 1 : 0][r] : yinn[a][r];
+          const T ry = b > 0 ? F[a][b > 0 ? b - 1 : 0][r]
+                             : (NY > 1 && wyi > 0) ? yr[r] : yinn[a][r];
           const T res = fmaT(rx, ldexpT(wz[r % PR::kWz], xe),
                              fmaT(ry, cy, fmaT(rz[r], cz, (T)vsc[b][r])));
           F[a][b][r] = res;
@@ -469,14 +494,22 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
         for (int b = 0; b < kTY; ++b) vsc[b] = vsn[b];
       }
     }
-    fetch_yin(t + 1);
+    if (wyi == 0) fetch_yin(t + 1);
+    // the last row of this row group is the next row group's y inflow
+    if (NY > 1 && wyi < NY - 1) {
+#pragma unroll
+      for (int a = 0; a < NAB; ++a)
+        store4(yring + ((((size_t)(t & 1) * (NY - 1) + wyi) * NWA + wa) * NAB + a) * 256 +
+                   lane * 4,
+               F[a][kTY - 1]);
+    }
 
     // ---- outflow faces for the downstream tiles ------------------------------
     if (x_out && j == wx - 1) {
 #pragma unroll
       for (int b = 0; b < kTY; ++b)
         if (val[b]) {
-          T* dst = fyz + (size_t)yb[b] * nze * nap + foff(t - j - b);
+          T* dst = fyz + (size_t)yb[b] * nze * nap + foff(t - j - b - kTY * wyi);
 #pragma unroll
           for (int a = 0; a < NAB; ++a) store4(dst + 16 * a, F[a][b]);
         }
@@ -484,32 +517,32 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
     if (y_out) {
 #pragma unroll
       for (int b = 0; b < kTY; ++b)
-        if (b == wy - 1 && val[b]) {
-          T* dst = fxz + (size_t)x * nze * nap + foff(t - j - b);
+        if (kTY * wyi + b == wy - 1 && val[b]) {
+          T* dst = fxz + (size_t)x * nze * nap + foff(t - j - b - kTY * wyi);
 #pragma unroll
           for (int a = 0; a < NAB; ++a) store4(dst + 16 * a, F[a][b]);
         }
     }
 
     // ---- vo: lane (j, g) holds moments 4g..4g+3 of cell (j, b) ---------------
-    A* rbuf = red + (size_t)(t & 1) * kTY * NWA * 64 * 4;
+    A* rbuf = red + ((size_t)(t & 1) * NY + wyi) * kTY * NWA * 64 * 4;
     if constexpr (NWA > 1) {
 #pragma unroll
       for (int b = 0; b < kTY; ++b)
-        if (b % NWA != wave) {
+        if (b % NWA != wa) {
           A v[4] = {vacc[b][0], vacc[b][1], vacc[b][2], vacc[b][3]};
-          store4(rbuf + ((size_t)(b * NWA + wave) * 64 + lane) * 4, v);
+          store4(rbuf + ((size_t)(b * NWA + wa) * 64 + lane) * 4, v);
         }
-      __syncthreads();
     }
+    if constexpr (NWA > 1 || NY > 1) __syncthreads();
 #pragma unroll
     for (int b = 0; b < kTY; ++b) {
-      if (b % NWA != wave) continue;
+      if (b % NWA != wa) continue;
       double sum[4];
 #pragma unroll
       for (int w = 0; w < NWA; ++w) {
         A v[4];
-        if (w == wave) {
+        if (w == wa) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) v[r] = vacc[b][r];
         } else {
@@ -521,9 +554,16 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
       }
       if (!first) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) sum[r] += von[b / NWA][r];
+        for (int r = 0; r < 4; ++r) sum[r] += (double)von[b / NWA][r];
       }
-      if (val[b] && 4 * g < NM) store4(p.vo + so[b] + 4 * g, sum);
+      if (val[b] && 4 * g < NM) {
+        if (kF64 || last) {
+          store4(p.vo + so[b] + 4 * g, sum);
+        } else {
+          float f[4] = {(float)sum[0], (float)sum[1], (float)sum[2], (float)sum[3]};
+          store4(p.vo32 + so[b] + 4 * g, f);
+        }
+      }
     }
     fetch_vo(zn, en);
 #pragma unroll
@@ -535,7 +575,7 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
 }
 
 // Workgroup bid of a launch -> (item, chunk of epw energy groups, unknown).
-template <class PR, int NM, int NAC, int NWA>
+template <class PR, int NM, int NAC, int NWA, int NY>
 __device__ __forceinline__ void sweep_block(const SweepParams& p, int item_base, int bid,
                                             int epw) {
   const int iu = bid % p.nu;
@@ -544,21 +584,21 @@ __device__ __forceinline__ void sweep_block(const SweepParams& p, int item_base,
   const int ec = bid % nec;
   bid /= nec;
   const int ie0 = ec * epw;
-  sweep_tile<PR, NM, NAC, NWA>(p, p.items[item_base + bid], ie0, min(epw, p.ne - ie0), iu);
+  sweep_tile<PR, NM, NAC, NWA, NY>(p, p.items[item_base + bid], ie0, min(epw, p.ne - ie0), iu);
 }
 
 // Single sweep.
-template <class PR, int NM, int NAC, int NWA>
-__global__ __launch_bounds__(64 * NWA) __attribute__((amdgpu_waves_per_eu(
+template <class PR, int NM, int NAC, int NWA, int NY>
+__global__ __launch_bounds__(64 * NWA * NY) __attribute__((amdgpu_waves_per_eu(
     Occupancy<PR, NAC>::value))) void k_sweep_mfma(SweepParams p, int item_base, int epw) {
-  sweep_block<PR, NM, NAC, NWA>(p, item_base, (int)blockIdx.x, epw);
+  sweep_block<PR, NM, NAC, NWA, NY>(p, item_base, (int)blockIdx.x, epw);
 }
 
 // Many sweeps in one launch: segment k of the launch is one launch of the
 // schedule of problem prob[k] (its items from item_base[k]); workgroups
 // [start[k], start[k+1]) belong to it.
-template <class PR, int NM, int NAC, int NWA>
-__global__ __launch_bounds__(64 * NWA) __attribute__((amdgpu_waves_per_eu(
+template <class PR, int NM, int NAC, int NWA, int NY>
+__global__ __launch_bounds__(64 * NWA * NY) __attribute__((amdgpu_waves_per_eu(
     Occupancy<PR, NAC>::value))) void k_sweep_mfma_multi(const SweepParams* __restrict__ probs,
                                                        MultiLaunch ml) {
   const int blk = (int)blockIdx.x;
@@ -566,7 +606,7 @@ __global__ __launch_bounds__(64 * NWA) __attribute__((amdgpu_waves_per_eu(
   for (int step = kMaxSegments / 2; step > 0; step >>= 1)
     if (k + step < ml.nseg && ml.start[k + step] <= blk) k += step;
   const SweepParams p = probs[ml.prob[k]];
-  sweep_block<PR, NM, NAC, NWA>(p, ml.item_base[k], blk - ml.start[k], ml.epw[k]);
+  sweep_block<PR, NM, NAC, NWA, NY>(p, ml.item_base[k], blk - ml.start[k], ml.epw[k]);
 }
 
 // Boundary inflow of every tile row and column: before the sweep, the x face
@@ -625,27 +665,29 @@ __global__ __launch_bounds__(256) void k_fill_faces_multi(const SweepParams* __r
   fill_row<PR>(p, blk - ml.start[k]);
 }
 
+// Workgroup shapes built: (waves of angle chunks, row groups of 4 rows), at
+// most 4 waves per workgroup.
+#define MSW_SHAPES(X) X(1, 1) X(2, 1) X(3, 1) X(4, 1) X(1, 2) X(2, 2) X(1, 4)
+
+bool mfma_shape_supported(int nwa, int ny) {
+#define MSW_CASE(W, Y) \
+  if (nwa == W && ny == Y) return true;
+  MSW_SHAPES(MSW_CASE)
+#undef MSW_CASE
+  return false;
+}
+
 template <class PR, int NM, int NAC>
 static void launch_nac(const LaunchSpec& ls, const SweepParams& sp, hipStream_t st) {
-  const dim3 block(64 * ls.nwa);
-  switch (ls.nwa) {
-    case 1:
-      hipLaunchKernelGGL((k_sweep_mfma<PR, NM, NAC, 1>), dim3(ls.grid), block, ls.lds_bytes,
-                         st, sp, ls.item_base, ls.epw);
-      break;
-    case 2:
-      hipLaunchKernelGGL((k_sweep_mfma<PR, NM, NAC, 2>), dim3(ls.grid), block, ls.lds_bytes,
-                         st, sp, ls.item_base, ls.epw);
-      break;
-    case 3:
-      hipLaunchKernelGGL((k_sweep_mfma<PR, NM, NAC, 3>), dim3(ls.grid), block, ls.lds_bytes,
-                         st, sp, ls.item_base, ls.epw);
-      break;
-    default:
-      hipLaunchKernelGGL((k_sweep_mfma<PR, NM, NAC, 4>), dim3(ls.grid), block, ls.lds_bytes,
-                         st, sp, ls.item_base, ls.epw);
-      break;
+  const dim3 block(64 * ls.nwa * ls.ny);
+#define MSW_CASE(W, Y)                                                                     \
+  if (ls.nwa == W && ls.ny == Y) {                                                         \
+    hipLaunchKernelGGL((k_sweep_mfma<PR, NM, NAC, W, Y>), dim3(ls.grid), block, ls.lds_bytes, \
+                       st, sp, ls.item_base, ls.epw);                                     \
+    return;                                                                                \
   }
+  MSW_SHAPES(MSW_CASE)
+#undef MSW_CASE
 }
 
 template <class PR, int NM>
@@ -685,59 +727,49 @@ void launch_mfma_fill(int precision, const SweepParams& sp, hipStream_t st) {
 }
 
 template <class PR, int NM, int NAC>
-static void launch_multi_nac(int nwa, unsigned grid, size_t lds, const SweepParams* probs,
+static void launch_multi_nac(int nwa, int ny, unsigned grid, size_t lds, const SweepParams* probs,
                              const MultiLaunch& ml, hipStream_t st) {
-  const dim3 block(64 * nwa);
-  switch (nwa) {
-    case 1:
-      hipLaunchKernelGGL((k_sweep_mfma_multi<PR, NM, NAC, 1>), dim3(grid), block, lds, st,
-                         probs, ml);
-      break;
-    case 2:
-      hipLaunchKernelGGL((k_sweep_mfma_multi<PR, NM, NAC, 2>), dim3(grid), block, lds, st,
-                         probs, ml);
-      break;
-    case 3:
-      hipLaunchKernelGGL((k_sweep_mfma_multi<PR, NM, NAC, 3>), dim3(grid), block, lds, st,
-                         probs, ml);
-      break;
-    default:
-      hipLaunchKernelGGL((k_sweep_mfma_multi<PR, NM, NAC, 4>), dim3(grid), block, lds, st,
-                         probs, ml);
-      break;
+  const dim3 block(64 * nwa * ny);
+#define MSW_CASE(W, Y)                                                                     \
+  if (nwa == W && ny == Y) {                                                               \
+    hipLaunchKernelGGL((k_sweep_mfma_multi<PR, NM, NAC, W, Y>), dim3(grid), block, lds, st, \
+                       probs, ml);                                                         \
+    return;                                                                                \
   }
+  MSW_SHAPES(MSW_CASE)
+#undef MSW_CASE
 }
 
 template <class PR, int NM>
-static void launch_multi_nm(int nac, int nwa, unsigned grid, size_t lds,
+static void launch_multi_nm(int nac, int nwa, int ny, unsigned grid, size_t lds,
                             const SweepParams* probs, const MultiLaunch& ml, hipStream_t st) {
   if (nac == 16)
-    launch_multi_nac<PR, NM, 16>(nwa, grid, lds, probs, ml, st);
+    launch_multi_nac<PR, NM, 16>(nwa, ny, grid, lds, probs, ml, st);
   else if (nac == 32)
-    launch_multi_nac<PR, NM, 32>(nwa, grid, lds, probs, ml, st);
+    launch_multi_nac<PR, NM, 32>(nwa, ny, grid, lds, probs, ml, st);
   else
-    launch_multi_nac<PR, NM, 64>(nwa, grid, lds, probs, ml, st);
+    launch_multi_nac<PR, NM, 64>(nwa, ny, grid, lds, probs, ml, st);
 }
 
 template <class PR>
-static void launch_multi_prec(int nm, int nac, int nwa, unsigned grid, size_t lds,
+static void launch_multi_prec(int nm, int nac, int nwa, int ny, unsigned grid, size_t lds,
                               const SweepParams* probs, const MultiLaunch& ml, hipStream_t st) {
   switch (nm) {
-    case 4: launch_multi_nm<PR, 4>(nac, nwa, grid, lds, probs, ml, st); break;
-    case 8: launch_multi_nm<PR, 8>(nac, nwa, grid, lds, probs, ml, st); break;
-    default: launch_multi_nm<PR, 16>(nac, nwa, grid, lds, probs, ml, st); break;
+    case 4: launch_multi_nm<PR, 4>(nac, nwa, ny, grid, lds, probs, ml, st); break;
+    case 8: launch_multi_nm<PR, 8>(nac, nwa, ny, grid, lds, probs, ml, st); break;
+    default: launch_multi_nm<PR, 16>(nac, nwa, ny, grid, lds, probs, ml, st); break;
   }
 }
 
-void launch_mfma_multi(int precision, int nm, int nac, int nwa, unsigned grid,
+void launch_mfma_multi(int precision, int nm, int nac, int nwa, int ny, unsigned grid,
                        const SweepParams* probs, const MultiLaunch& ml, hipStream_t st) {
-  const size_t lds = mfma_lds_bytes(precision, nac, nwa);
+  const size_t lds = mfma_lds_bytes(precision, nac, nwa, ny);
   if (precision == MSW_PREC_MIXED_FP32)
-    launch_multi_prec<PrecF32>(nm, nac, nwa, grid, lds, probs, ml, st);
+    launch_multi_prec<PrecF32>(nm, nac, nwa, ny, grid, lds, probs, ml, st);
   else if (precision == MSW_PREC_MIXED_BF16)
-    launch_multi_prec<PrecBF16>(nm, nac, nwa, grid, lds, probs, ml, st);
+    launch_multi_prec<PrecBF16>(nm, nac, nwa, ny, grid, lds, probs, ml, st);
   else
-    launch_multi_prec<PrecF64>(nm, nac, nwa, grid, lds, probs, ml, st);
+    launch_multi_prec<PrecF64>(nm, nac, nwa, ny, grid, lds, probs, ml, st);
 }
 
 void launch_mfma_fill_multi(int precision, unsigned grid, const SweepParams* probs,
@@ -754,13 +786,15 @@ size_t mfma_face_elem_bytes(int precision) {
   return precision == MSW_PREC_FP64 ? sizeof(double) : sizeof(float);
 }
 
-size_t mfma_lds_bytes(int precision, int nac, int nwa) {
+size_t mfma_lds_bytes(int precision, int nac, int nwa, int ny) {
   const size_t t = mfma_face_elem_bytes(precision);  // accumulator = face type
-  const size_t red = nwa > 1 ? 2 * (size_t)kTY * nwa * 64 * 4 * t : 0;
-  return red + (size_t)nwa * (nac / 16) * 64 * t;
+  const size_t nab = nac / 16;
+  const size_t red = nwa > 1 ? 2 * (size_t)ny * kTY * nwa * 64 * 4 * t : 0;
+  const size_t ring = ny > 1 ? 2 * (size_t)(ny - 1) * nwa * nab * 64 * 4 * t : 0;
+  return red + ring + (size_t)ny * nwa * nab * 64 * t;
 }
 
 int mfma_tile_x() { return kTX; }
-int mfma_tile_y() { return kTY; }
+int mfma_tile_rows() { return kTY; }
 
 }  // namespace msw

@@ -65,6 +65,18 @@ int default_mfma_epw(int nz, int ne) {
   return e < 1 ? 1 : e > ne ? ne : e;
 }
 
+// Row groups of 4 rows per MFMA tile.  Measured at 64^3, na=64: FP64 (32
+// angles per wave, two waves per tile) is fastest with one row group (the
+// per-step barrier of four waves costs more than the y-face traffic it
+// saves); the mixed forms (64 angles per wave) with two.  Never more rows
+// than the grid has.
+int default_mfma_ny(int precision, int nwa, int ncell_y) {
+  const int want = precision == MSW_PREC_FP64 ? 1 : 2;
+  for (int ny = want; ny > 1; ny /= 2)
+    if (mfma_shape_supported(nwa, ny) && 4 * ny <= ((ncell_y + 3) / 4) * 4) return ny;
+  return 1;
+}
+
 int launch_mfma_epw(int preferred, int ne, long long waves1) {
   int e = preferred < 1 ? 1 : preferred > ne ? ne : preferred;
   while (e > 1 && waves1 * ((ne + e - 1) / e) < 4096) --e;
@@ -307,7 +319,7 @@ __global__ __launch_bounds__(512) void k_sweep_fp64(SweepParams p,
     if (nwa == 1) {
       if (active) {
         double2* dst = reinterpret_cast<double2*>(p.vo + cbase);
-        if (it.first) {
+        if (it.flags & kVisitFirst) {
 #pragma unroll
           for (int i = 0; i < NM / 2; ++i) dst[i] = make_double2(acc[2 * i], acc[2 * i + 1]);
         } else {
@@ -330,7 +342,7 @@ __global__ __launch_bounds__(512) void k_sweep_fp64(SweepParams p,
           double sum = red[(size_t)im * 64 + lane];
           for (int w = 1; w < nwa; ++w) sum += red[((size_t)w * NM + im) * 64 + lane];
           double* dst = p.vo + cbase + im;
-          *dst = it.first ? sum : *dst + sum;
+          *dst = (it.flags & kVisitFirst) ? sum : *dst + sum;
         }
       }
       __syncthreads();
@@ -435,13 +447,13 @@ __global__ __launch_bounds__(kNormThreads) void k_norms_final(
 // that launch.  Octant priority: kind 0, most remaining visits first; kind 1,
 // the reference's fixed octant order {0,4,2,6,3,7,1,5}
 // (src/3_sweeper/stepscheduler_kba.c:155).  Returns the launches in order;
-// marks the first visit of every physical tile.
+// marks the first and the last visit of every physical tile.
 std::vector<std::vector<TileItem>> build_schedule(int ntx, int nty, int kind) {
   static const int kRefOrder[kNoct] = {0, 4, 2, 6, 3, 7, 1, 5};
   std::vector<std::vector<TileItem>> launches;
   std::vector<std::vector<int>> done(kNoct, std::vector<int>(ntx * nty, 0));
   std::vector<int> remaining(kNoct, ntx * nty);
-  std::vector<int> touched(ntx * nty, 0);
+  std::vector<int> touched(ntx * nty, 0);  // visits so far per physical tile
   int left = kNoct * ntx * nty;
   while (left > 0) {
     std::vector<int> order(kNoct);
@@ -469,7 +481,7 @@ std::vector<std::vector<TileItem>> build_schedule(int ntx, int nty, int kind) {
           t.octant = o;
           t.tx = i;
           t.ty = j;
-          t.first = touched[g] ? 0 : 1;
+          t.flags = (touched[g] == 0 ? kVisitFirst : 0) | (touched[g] == kNoct - 1 ? kVisitLast : 0);
           items.push_back(t);
         }
     }
@@ -477,7 +489,7 @@ std::vector<std::vector<TileItem>> build_schedule(int ntx, int nty, int kind) {
       done[t.octant][t.tx + ntx * t.ty] = 2;  // finished in this launch
       const int gx = (t.octant & 1) ? ntx - 1 - t.tx : t.tx;
       const int gy = (t.octant & 2) ? nty - 1 - t.ty : t.ty;
-      touched[gx + ntx * gy] = 1;
+      touched[gx + ntx * gy]++;
       remaining[t.octant]--;
       left--;
     }
@@ -521,12 +533,14 @@ struct MswPlan {
   int precision;
   int engine;  // MSW_ENGINE_*
   int nac, nwa, na_pad, ntx, nty;
+  int ny;  // row groups per tile (MFMA engine)
   std::vector<int> launch_epw;  // energy groups per workgroup of each launch (MFMA)
   size_t lds_bytes;
   double* at;  // a_from_m transposed [8][na][nm] (+ padding)
   double* mf;  // m_from_a copy       [8][na][nm] (+ padding)
   void* fyz;   // face planes (element type of the precision)
   void* fxz;
+  float* vo32;  // mixed forms: FP32 running vo sum over the octants
   double* norm_part;
   TileItem* items;
   std::vector<int> launch_off, launch_cnt;
@@ -549,6 +563,7 @@ static int launch_all(MswPlan* p, const SweepParams& sp, hipStream_t st) {
       ls.item_base = p->launch_off[l];
       ls.grid = grid.x;
       ls.nwa = p->nwa;
+      ls.ny = p->ny;
       ls.nac = p->nac;
       ls.nm = p->d.nm;
       ls.lds_bytes = p->lds_bytes;
@@ -611,6 +626,7 @@ void msw_plan_default_options(MswPlanOptions* o) {
   o->engine = MSW_ENGINE_MFMA;
   o->angles_per_wave = 0;
   o->groups_per_workgroup = 0;
+  o->tile_rows = 0;
 }
 
 int msw_plan_create(const MswDims* d, int precision, MswPlan** out) {
@@ -672,11 +688,23 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
   }
   // tile of one wave: 8x8 cells (VALU engine), 16x4 (MFMA engine)
   const int tx = p->engine == MSW_ENGINE_MFMA ? mfma_tile_x() : kTile;
-  const int ty = p->engine == MSW_ENGINE_MFMA ? mfma_tile_y() : kTile;
+  p->ny = 1;
+  if (p->engine == MSW_ENGINE_MFMA) {
+    p->ny = opt->tile_rows > 0 ? opt->tile_rows / mfma_tile_rows()
+                               : default_mfma_ny(precision, p->nwa, d->ncell_y);
+    if (opt->tile_rows % mfma_tile_rows() != 0 || !mfma_shape_supported(p->nwa, p->ny)) {
+      delete p;
+      return set_err(MSW_ERR_UNSUPPORTED,
+                     "tile_rows %s%ld is not built for this angle chunking (4, 8 or 16 with at "
+                     "most 4 waves per workgroup)",
+                     "", opt->tile_rows);
+    }
+  }
+  const int ty = p->engine == MSW_ENGINE_MFMA ? mfma_tile_rows() * p->ny : kTile;
   p->ntx = (d->ncell_x + tx - 1) / tx;
   p->nty = (d->ncell_y + ty - 1) / ty;
   if (p->engine == MSW_ENGINE_MFMA)
-    p->lds_bytes = mfma_lds_bytes(precision, p->nac, p->nwa);
+    p->lds_bytes = mfma_lds_bytes(precision, p->nac, p->nwa, p->ny);
   else
     p->lds_bytes = sizeof(double) * ((size_t)p->nwa * 2 * kTile * (p->nac + 1) +
                                      (p->nwa > 1 ? (size_t)p->nwa * d->nm * 64 : 0));
@@ -711,6 +739,10 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
   if (e == hipSuccess) e = hipMemset(p->mf, 0, sizeof(double) * nat);
   if (e == hipSuccess) e = hipMalloc(&p->fyz, fsz * nfyz);
   if (e == hipSuccess) e = hipMalloc(&p->fxz, fsz * nfxz);
+  if (e == hipSuccess && p->engine == MSW_ENGINE_MFMA && precision != MSW_PREC_FP64) {
+    e = hipMalloc(&p->vo32, sizeof(float) * msw_size_state(d));
+    p->bytes += sizeof(float) * msw_size_state(d);
+  }
   if (e == hipSuccess) e = hipMalloc(&p->norm_part, sizeof(double) * 2 * kNormBlocks);
   if (e == hipSuccess) e = hipMalloc(&p->items, sizeof(TileItem) * flat.size());
   if (e == hipSuccess)
@@ -772,6 +804,7 @@ int msw_plan_destroy(MswPlan* p) {
   if (p->mf) (void)hipFree(p->mf);
   if (p->fyz) (void)hipFree(p->fyz);
   if (p->fxz) (void)hipFree(p->fxz);
+  if (p->vo32) (void)hipFree(p->vo32);
   if (p->norm_part) (void)hipFree(p->norm_part);
   if (p->items) (void)hipFree(p->items);
   delete p;
@@ -795,6 +828,7 @@ int msw_plan_sweep(MswPlan* p, const double* vi, double* vo,
   sp.mf = p->mf;
   sp.fyz = p->fyz;
   sp.fxz = p->fxz;
+  sp.vo32 = p->vo32;
   sp.items = p->items;
   sp.nx = d.ncell_x;
   sp.ny = d.ncell_y;

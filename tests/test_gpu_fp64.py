@@ -153,3 +153,46 @@ def test_energy_streaming_vs_oracle(shape, cuda):
             assert_close(outs[0], ref, what="vo vs oracle")
         for o in outs[1:]:
             np.testing.assert_array_equal(o, outs[0])
+
+
+ROWS_SHAPES = [(9, 7, 3, 2, 16, 64, 4), (12, 20, 5, 2, 16, 32, 4), (5, 9, 4, 3, 4, 40, 2),
+               (20, 12, 4, 2, 16, 32, 2), (34, 18, 3, 3, 8, 16, 1)]
+
+
+@pytest.mark.parametrize("shape", ROWS_SHAPES, ids=[str(s) for s in ROWS_SHAPES])
+def test_tile_rows_vs_oracle(shape, cuda):
+    """Tiles of 4, 8 and 16 rows (row groups handing y faces over in LDS),
+    partial tiles included: FP64 within the parity bar, and the same bits for
+    every tile height, FP64 and mixed FP32 (32 angles per wave, so that 8-row
+    tiles run two angle chunks by two row groups)."""
+    import torch
+    print(f"case {shape}", flush=True)
+    d = msw.dims(*shape)
+    od = oracle.dims(*shape)
+    rng = np.random.default_rng(23)
+    base = oracle.init_state(od)
+    vi0 = base * (1.0 + 0.25 * rng.uniform(-1, 1, base.size))
+    a, m = oracle.init_matrices(od)
+    ref = oracle.sweep(od, a, m, vi0)
+    ta, tm = torch.from_numpy(a).to(cuda), torch.from_numpy(m).to(cuda)
+    vi = torch.from_numpy(vi0).to(cuda)
+    for prec in (msw.PREC_FP64, msw.PREC_MIXED_FP32):
+        outs = []
+        for rows in (4, 8, 16):
+            nac = 32 if rows <= 8 else 64
+            vo = torch.full_like(vi, float("nan"))
+            plan = msw.Plan(d, prec, engine=msw.ENGINE_MFMA, angles_per_wave=nac, tile_rows=rows)
+            plan.sweep(vi, vo, ta, tm)
+            torch.cuda.synchronize()
+            plan.close()
+            outs.append(vo.cpu().numpy())
+        if prec == msw.PREC_FP64:
+            for o in outs:
+                assert_close(o, ref, what="vo vs oracle")
+        for o in outs[1:]:
+            if prec == msw.PREC_MIXED_FP32:
+                # chunking (32 vs 64 angles per wave) changes only the order of
+                # the chunk sum
+                assert np.max(np.abs(o - outs[0])) <= 1e-5 * np.max(np.abs(outs[0]))
+            else:
+                assert_close(o, outs[0], what="tile heights")
