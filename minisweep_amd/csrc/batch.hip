@@ -132,16 +132,24 @@ int make_geom(const MswDims& d, int nac, Geom* g, int ny = 0, int precision = MS
   return MSW_OK;
 }
 
-// FP32 running vo sum of the mixed forms (none in FP64).
+// FP32 running vo sum of the mixed forms (none in FP64), followed in the
+// BF16 form by the FP32 copy of vi.
 size_t vo32_bytes(const Geom& g, int precision) {
-  return precision == MSW_PREC_FP64 ? 0 : align_up(sizeof(float) * g.state_doubles());
+  const size_t one = align_up(sizeof(float) * g.state_doubles());
+  return precision == MSW_PREC_FP64 ? 0 : precision == MSW_PREC_MIXED_BF16 ? 2 * one : one;
+}
+const float* vi32_of(const Geom& g, float* vo32, int precision) {
+  return vo32 && precision == MSW_PREC_MIXED_BF16
+             ? (const float*)((char*)vo32 + align_up(sizeof(float) * g.state_doubles()))
+             : nullptr;
 }
 
 SweepParams make_params(const Geom& g, const double* vi, double* vo, const double* at,
                         const double* mf, void* fyz, void* fxz, float* vo32,
-                        const DevSchedule* s) {
+                        const DevSchedule* s, int precision) {
   SweepParams sp;
   sp.vo32 = vo32;
+  sp.vi32 = vi32_of(g, vo32, precision);
   sp.vi = vi;
   sp.vo = vo;
   sp.at = at;
@@ -351,12 +359,17 @@ int batch_enqueue(MswBatch* b, hipStream_t st) {
   for (auto& p : b->probs) {
     launch_pack(p.afm, p.mfa, p.at, p.mf, p.g.d.nm, p.g.d.na, p.g.nap, st);
     ++nk;
+    if (const float* v32 = vi32_of(p.g, p.vo32, b->opt.precision)) {
+      launch_to_f32(p.vi, (float*)v32, p.g.state_doubles(), st);
+      ++nk;
+    }
   }
   std::vector<SweepParams> hp(b->probs.size());
   std::vector<std::pair<int, const SweepParams*>> fill;
   for (size_t i = 0; i < b->probs.size(); ++i) {
     const BatchProb& p = b->probs[i];
-    hp[i] = make_params(p.g, p.vi, p.vo, p.at, p.mf, p.fyz, p.fxz, p.vo32, p.s);
+    hp[i] = make_params(p.g, p.vi, p.vo, p.at, p.mf, p.fyz, p.fxz, p.vo32, p.s,
+                        b->opt.precision);
   }
   for (size_t i = 0; i < b->probs.size(); ++i) fill.emplace_back((int)i, &hp[i]);
   nk += enqueue_fill(prec, fill, b->d_params, st);
@@ -422,7 +435,8 @@ int batch_build(MswBatch* b) {
   std::vector<SweepParams> hp(b->probs.size());
   for (size_t i = 0; i < b->probs.size(); ++i) {
     const BatchProb& p = b->probs[i];
-    hp[i] = make_params(p.g, p.vi, p.vo, p.at, p.mf, p.fyz, p.fxz, p.vo32, p.s);
+    hp[i] = make_params(p.g, p.vi, p.vo, p.at, p.mf, p.fyz, p.fxz, p.vo32, p.s,
+                        b->opt.precision);
   }
   HIP_OK(hipMalloc(&b->d_params, sizeof(SweepParams) * hp.size()), "batch params");
   HIP_OK(hipMemcpy(b->d_params, hp.data(), sizeof(SweepParams) * hp.size(), hipMemcpyHostToDevice),
@@ -925,7 +939,7 @@ void server_loop(MswServer* s) {
       for (size_t k : fresh) {
         Active& a = s->active[k];
         s->h_params[a.slot] =
-            make_params(a.g, a.vi, a.vo, a.at, a.mf, a.fyz, a.fxz, a.vo32, a.s);
+            make_params(a.g, a.vi, a.vo, a.at, a.mf, a.fyz, a.fxz, a.vo32, a.s, s->opt.precision);
         (void)hipMemcpyAsync(s->d_params + a.slot, s->h_params + a.slot, sizeof(SweepParams),
                              hipMemcpyHostToDevice, s->st);
         if (a.req.vi_pinned)
@@ -933,6 +947,8 @@ void server_loop(MswServer* s) {
                                hipMemcpyHostToDevice, s->st);
         else
           launch_init_state(a.g.d, a.vi, s->st);
+        if (const float* v32 = vi32_of(a.g, a.vo32, s->opt.precision))
+          launch_to_f32(a.vi, (float*)v32, a.g.state_doubles(), s->st);
         fill.emplace_back(a.slot, &s->h_params[a.slot]);
         a.started = true;
       }

@@ -70,6 +70,7 @@ void launch_pack(const double* afm, const double* mfa, double* at, double* mf, i
                  int na_pad, hipStream_t st);
 void launch_init_state(const MswDims& d, double* v, hipStream_t st);
 size_t norms_scratch_doubles();
+void launch_to_f32(const double* src, float* dst, size_t n, hipStream_t st);
 void launch_norms(const double* vi, const double* vo, size_t n, double* part, double* out2,
                   hipStream_t st);
 

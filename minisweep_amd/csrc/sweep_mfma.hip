@@ -339,12 +339,15 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
       en[b] = wrap ? es[b] + 1 : es[b];
     }
   };
+  // the input state: FP64, or its FP32 copy in the BF16 form
+  typedef typename std::conditional<PR::kPacked, float, double>::type VI;
+  const VI* vsrc = PR::kPacked ? (const VI*)p.vi32 : (const VI*)p.vi;
   auto fetch_vi = [&](const int (&ze)[kTY], const int (&ee)[kTY]) {
 #pragma unroll
     for (int b = 0; b < kTY; ++b) {
-      const double* src = p.vi + soff(b, ee[b], ze[b]) + moff;
+      const VI* src = vsrc + soff(b, ee[b], ze[b]) + moff;
       if constexpr (KA == 4) {
-        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        VI v[4] = {0, 0, 0, 0};
         if (!mpad) load4(src, v);
 #pragma unroll
         for (int s = 0; s < 4; ++s) vin[b][s] = (T)v[s];

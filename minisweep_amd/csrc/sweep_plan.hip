@@ -367,6 +367,18 @@ __global__ void k_pack_matrices(const double* __restrict__ afm,
 }
 
 // Manufactured state v = (1+im) S(x,y,z) E(ie) U(iu), layout [iz][ie][iy][ix][iu][im].
+// FP64 -> FP32 copy of a state (the BF16 form's input, once per sweep), 4
+// values per thread and step; n is a multiple of 4 (nm is).
+__global__ __launch_bounds__(256) void k_state_to_f32(const double* __restrict__ src,
+                                                      float* __restrict__ dst, size_t n) {
+  for (size_t i = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) * 4; i < n;
+       i += (size_t)gridDim.x * blockDim.x * 4) {
+    const double2 a = *reinterpret_cast<const double2*>(src + i);
+    const double2 b = *reinterpret_cast<const double2*>(src + i + 2);
+    *reinterpret_cast<float4*>(dst + i) = float4{(float)a.x, (float)a.y, (float)b.x, (float)b.y};
+  }
+}
+
 __global__ void k_init_state(double* __restrict__ v, int nx, int ny, int nz,
                              int ne, int nm, int nu, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
@@ -517,6 +529,11 @@ void launch_init_state(const MswDims& d, double* v, hipStream_t st) {
 
 size_t norms_scratch_doubles() { return 2 * (size_t)kNormBlocks; }
 
+void launch_to_f32(const double* src, float* dst, size_t n, hipStream_t st) {
+  const unsigned blocks = (unsigned)std::min<size_t>((n / 4 + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_state_to_f32, dim3(blocks), dim3(256), 0, st, src, dst, n);
+}
+
 void launch_norms(const double* vi, const double* vo, size_t n, double* part, double* out2,
                   hipStream_t st) {
   hipLaunchKernelGGL(k_norms_partial, dim3(kNormBlocks), dim3(kNormThreads), 0, st, vi, vo, n,
@@ -541,6 +558,7 @@ struct MswPlan {
   void* fyz;   // face planes (element type of the precision)
   void* fxz;
   float* vo32;  // mixed forms: FP32 running vo sum over the octants
+  float* vi32;  // mixed forms: FP32 copy of vi
   double* norm_part;
   TileItem* items;
   std::vector<int> launch_off, launch_cnt;
@@ -742,6 +760,12 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
   if (e == hipSuccess && p->engine == MSW_ENGINE_MFMA && precision != MSW_PREC_FP64) {
     e = hipMalloc(&p->vo32, sizeof(float) * msw_size_state(d));
     p->bytes += sizeof(float) * msw_size_state(d);
+    // the BF16 form reads an FP32 copy of vi (measured faster at 64^3); the
+    // FP32 form reads vi itself, the copy would cost more than it saves
+    if (e == hipSuccess && precision == MSW_PREC_MIXED_BF16) {
+      e = hipMalloc(&p->vi32, sizeof(float) * msw_size_state(d));
+      p->bytes += sizeof(float) * msw_size_state(d);
+    }
   }
   if (e == hipSuccess) e = hipMalloc(&p->norm_part, sizeof(double) * 2 * kNormBlocks);
   if (e == hipSuccess) e = hipMalloc(&p->items, sizeof(TileItem) * flat.size());
@@ -805,6 +829,7 @@ int msw_plan_destroy(MswPlan* p) {
   if (p->fyz) (void)hipFree(p->fyz);
   if (p->fxz) (void)hipFree(p->fxz);
   if (p->vo32) (void)hipFree(p->vo32);
+  if (p->vi32) (void)hipFree(p->vi32);
   if (p->norm_part) (void)hipFree(p->norm_part);
   if (p->items) (void)hipFree(p->items);
   delete p;
@@ -829,6 +854,8 @@ int msw_plan_sweep(MswPlan* p, const double* vi, double* vo,
   sp.fyz = p->fyz;
   sp.fxz = p->fxz;
   sp.vo32 = p->vo32;
+  sp.vi32 = p->vi32;
+  if (p->vi32) launch_to_f32(vi, p->vi32, msw_size_state(&d), st);
   sp.items = p->items;
   sp.nx = d.ncell_x;
   sp.ny = d.ncell_y;
