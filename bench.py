@@ -36,7 +36,8 @@ METRIC = ("sweep GFLOP/s + p50/p99 latency, mixed vs FP64 plain at ≤1e-5 rel e
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 TRAFFIC_FILES = {"fp64": "k_sweep_mfma_f64_64cube_na64.md",
-                 "fp32": "k_sweep_mfma_mixed_fp32_64cube_na64.md"}
+                 "fp32": "k_sweep_mfma_mixed_fp32_64cube_na64.md",
+                 "bf16": "k_sweep_mfma_mixed_bf16_64cube_na64.md"}
 
 
 def closed_form_normsq(ncx, ncy, ncz, ne, nm, nu):
