@@ -180,17 +180,18 @@ struct TickWork {
 
 // Energy groups per workgroup for a tick: each problem's preferred stream
 // length, halved for all while the tick would have < 4 waves per SIMD.
-void choose_tick_epw(std::vector<TickWork>& work) {
+void choose_tick_epw(std::vector<TickWork>& work, int precision) {
+  const long long target = precision == MSW_PREC_FP64 ? 4096 : 2048;
   for (TickWork& w : work) w.epw = w.g->epw;
   for (;;) {
     long long waves = 0;
     int emax = 1;
     for (const TickWork& w : work) {
       waves += (long long)w.s->cnt[w.launch] * ((w.g->d.ne + w.epw - 1) / w.epw) * w.g->d.nu *
-               w.g->nwa;
+               w.g->nwa * w.g->ny;
       emax = std::max(emax, w.epw);
     }
-    if (waves >= 4096 || emax == 1) return;
+    if (waves >= target || emax == 1) return;
     for (TickWork& w : work) w.epw = std::max(1, w.epw / 2);
   }
 }
@@ -199,7 +200,7 @@ void choose_tick_epw(std::vector<TickWork>& work) {
 // of up to kMaxSegments problems each.  Returns the number of launches.
 int enqueue_tick(int precision, std::vector<TickWork>& work, const SweepParams* d_params,
                  hipStream_t st) {
-  choose_tick_epw(work);
+  choose_tick_epw(work, precision);
   std::map<std::tuple<int, int, int, int>, std::vector<const TickWork*>> by_variant;
   for (const TickWork& w : work) by_variant[w.g->variant()].push_back(&w);
   int nl = 0;

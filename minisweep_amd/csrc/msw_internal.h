@@ -57,7 +57,7 @@ int default_mfma_nac(int precision, int na);
 // per workgroup, then fewer while a launch of `waves1` waves at one group per
 // workgroup would drop below 4 waves per SIMD.
 int default_mfma_epw(int nz, int ne);
-int launch_mfma_epw(int preferred, int ne, long long waves1);
+int launch_mfma_epw(int preferred, int ne, long long waves1, int precision);
 // Default row groups per MFMA tile.
 int default_mfma_ny(int precision, int nwa, int ncell_y);
 

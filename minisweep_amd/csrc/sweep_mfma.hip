@@ -81,6 +81,40 @@ __device__ __forceinline__ float shr1_keep(float old, float v) {
                                                     kRowShr1, 0xF, 0xF, false));
 }
 
+// Lane i of each 16-lane row reads lane (i - k) mod 16 (row_ror:k, k in 1..15).
+template <int K>
+__device__ __forceinline__ float row_ror(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x120 + K, 0xF, 0xF, false));
+}
+template <int K>
+__device__ __forceinline__ double row_ror(double v) {
+  return __hiloint2double(__builtin_amdgcn_mov_dpp(__double2hiint(v), 0x120 + K, 0xF, 0xF, false),
+                          __builtin_amdgcn_mov_dpp(__double2loint(v), 0x120 + K, 0xF, 0xF, false));
+}
+// Lane 0 of each row gets the row's lane s (s in 0..15, a constant after
+// unrolling); other lanes get whatever the rotation brings (unused).
+template <class T>
+__device__ __forceinline__ T from_lane(T v, int s) {
+  switch (s) {
+    case 1: return row_ror<15>(v);
+    case 2: return row_ror<14>(v);
+    case 3: return row_ror<13>(v);
+    case 4: return row_ror<12>(v);
+    case 5: return row_ror<11>(v);
+    case 6: return row_ror<10>(v);
+    case 7: return row_ror<9>(v);
+    case 8: return row_ror<8>(v);
+    case 9: return row_ror<7>(v);
+    case 10: return row_ror<6>(v);
+    case 11: return row_ror<5>(v);
+    case 12: return row_ror<4>(v);
+    case 13: return row_ror<3>(v);
+    case 14: return row_ror<2>(v);
+    case 15: return row_ror<1>(v);
+    default: return v;
+  }
+}
+
 __device__ __forceinline__ double fmaT(double a, double b, double c) {
   return __builtin_fma(a, b, c);
 }
@@ -112,12 +146,16 @@ __device__ __forceinline__ void store4(float* d, const float (&v)[4]) {
 // T: face / solve / operand type; A: MFMA accumulator element; row(g, r): the
 // angle (within a 16-angle block) that MFMA result register r of lane group g
 // holds; mom(i): the moment that a2m result row i stands for, chosen so that
-// lane group g, register r holds moment 4g + r.
+// lane group g, register r holds moment 4g + r.  kPingPong: the z loop
+// alternates two operand buffers (two steps per iteration) instead of copying
+// the prefetched operands at the top of each step; measured faster for the
+// mixed forms, slower for FP64 (register pressure).
 struct PrecF64 {
   typedef double T;
   typedef double A;
   typedef d4v acc4;
   static constexpr bool kPacked = false;
+  static constexpr bool kPingPong = false;
   static constexpr int kWz = 2;  // (g + 4r) & 7 depends on r & 1 only
   static __device__ __forceinline__ constexpr int row(int g, int r) { return g + 4 * r; }
   static __device__ __forceinline__ constexpr int mom(int i) { return 4 * (i & 3) + (i >> 2); }
@@ -130,6 +168,7 @@ struct PrecF32 {
   typedef float A;
   typedef f4v acc4;
   static constexpr bool kPacked = false;
+  static constexpr bool kPingPong = true;
   static constexpr int kWz = 4;
   static __device__ __forceinline__ constexpr int row(int g, int r) { return 4 * g + r; }
   static __device__ __forceinline__ constexpr int mom(int i) { return i; }
@@ -142,6 +181,7 @@ struct PrecBF16 {
   typedef float A;
   typedef f4v acc4;
   static constexpr bool kPacked = true;  // K = 16: 4 values per lane per MFMA
+  static constexpr bool kPingPong = true;
   static constexpr int kWz = 4;
   static __device__ __forceinline__ constexpr int row(int g, int r) { return 4 * g + r; }
   static __device__ __forceinline__ constexpr int mom(int i) { return i; }
@@ -240,12 +280,10 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
   const int zb = dz > 0 ? -1 : nz;
 
   // ---- LDS: vo partial sums [2][NY][kTY][NWA][64][4], y faces between row
-  // groups [2][NY-1][NWA][NAB][64][4], x inflow staging [NY*NWA][NAB][64] --
+  // groups [2][NY-1][NWA][NAB][64][4] --
   extern __shared__ double lds_raw[];
   A* red = reinterpret_cast<A*>(lds_raw);
   T* yring = reinterpret_cast<T*>(red + (NWA > 1 ? 2 * NY * kTY * NWA * 64 * 4 : 0));
-  T* xs_lds = yring + (NY > 1 ? 2 * (NY - 1) * NWA * NAB * 64 * 4 : 0) +
-              (size_t)wave * NAB * 64;
 
   // ---- face planes of this (octant, unknown): [y or x][group][z][angle],
   // z in the octant's frame so that a stream position maps to one offset --
@@ -298,12 +336,16 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
   // right after its last use in this one.  Every prefetch is issued on every
   // path, the last step's too (z is clamped), so that the memory counters
   // agree across paths and each wait covers only the data it needs. ---------
-  T vin[kTY][KA];        // m2a B operand
+  // m2a B operand, two buffers: one step's while the next step's loads are in
+  // flight (a whole step ahead)
+  T vbA[kTY][KA], vbB[kTY][KA];
   // running vo sum of the owned blocks' cells from earlier octant visits:
   // FP64 vo in the FP64 form, the FP32 sum in the mixed forms
   typedef typename std::conditional<kF64, double, float>::type V;
   V von[NOWN][4];
-  T xsn[NAB];            // x inflow, spread: lane (j, g) holds block j/4, register j%4
+  // x inflow, spread: lane (j, g) holds block j/4, register j%4, and lane
+  // (0, g) gathers it by DPP; two buffers as for vi
+  T xbA[NAB], xbB[NAB];
   T yinn[NAB][4];        // y inflow of block 0
   const int ns = nge * nz;  // stream length of this workgroup
   const int nsteps = ns + kTX + kTYT - 2;
@@ -342,7 +384,7 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
   // the input state: FP64, or its FP32 copy in the BF16 form
   typedef typename std::conditional<PR::kPacked, float, double>::type VI;
   const VI* vsrc = PR::kPacked ? (const VI*)p.vi32 : (const VI*)p.vi;
-  auto fetch_vi = [&](const int (&ze)[kTY], const int (&ee)[kTY]) {
+  auto fetch_vi = [&](const int (&ze)[kTY], const int (&ee)[kTY], T (&vnx)[kTY][KA]) {
 #pragma unroll
     for (int b = 0; b < kTY; ++b) {
       const VI* src = vsrc + soff(b, ee[b], ze[b]) + moff;
@@ -350,21 +392,24 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
         VI v[4] = {0, 0, 0, 0};
         if (!mpad) load4(src, v);
 #pragma unroll
-        for (int s = 0; s < 4; ++s) vin[b][s] = (T)v[s];
+        for (int s = 0; s < 4; ++s) vnx[b][s] = (T)v[s];
       } else {
 #pragma unroll
-        for (int s = 0; s < KA; ++s) vin[b][s] = (T)src[s];
+        for (int s = 0; s < KA; ++s) vnx[b][s] = (T)src[s];
       }
     }
   };
+  // running sum: vo (FP64 form) or vo32 (mixed).  A tile's first visit has
+  // no sum to read; it loads 16 bytes of the octant matrices instead (cached,
+  // never used), so the load is still issued on every path.
+  const V* vrd = first ? (const V*)p.at : kF64 ? (const V*)p.vo : (const V*)p.vo32;
   auto fetch_vo = [&](const int (&ze)[kTY], const int (&ee)[kTY]) {
 #pragma unroll
     for (int b = 0; b < kTY; ++b)
       if (b % NWA == wa && 4 * g < NM)
-        load4((kF64 ? (const V*)p.vo : (const V*)p.vo32) + soff(b, ee[b], ze[b]) + 4 * g,
-              von[b / NWA]);
+        load4(vrd + (first ? 0 : soff(b, ee[b], ze[b])) + 4 * g, von[b / NWA]);
   };
-  auto fetch_xs = [&](int t) {
+  auto fetch_xs = [&](int t, T (&xsn)[NAB]) {
     const int bb = j >> 2;
     const int r = kTY * wyi + bb;
     const int bc = r < wy ? r : wy - 1;
@@ -379,20 +424,18 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
     for (int a = 0; a < NAB; ++a) load4(src + 16 * a, yinn[a]);
   };
 
-  fetch_xs(0);
-  fetch_vi(zs, es);
+  fetch_xs(0, PR::kPingPong ? xbA : xbB);
+  fetch_vi(zs, es, PR::kPingPong ? vbA : vbB);
   if (wyi == 0) fetch_yin(0);
   fetch_vo(zs, es);
 
-  for (int t = 0; t < nsteps; ++t) {
+  // One step: this step's operands are vin and xsc, the next step's are
+  // fetched into vnx and xsn first thing.
+  auto step = [&](const int t, const T (&vin)[kTY][KA], T (&vnx)[kTY][KA], const T (&xsc)[NAB],
+                  T (&xsn)[NAB]) {
     advance();
-    // x inflow of this step into LDS: lane (0, g) reads its 16 values back
-#pragma unroll
-    for (int a = 0; a < NAB; ++a) xs_lds[a * 64 + lane] = xsn[a];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    fetch_xs(t + 1);
+    fetch_vi(zn, en, vnx);
+    fetch_xs(t + 1, xsn);
 
     // ---- per cell block: validity, z, spatial phase; z inflow on entry -------
     // A lane whose cell starts a group (z = 0) takes the domain-boundary z
@@ -437,13 +480,9 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
     // matrix cores work while the vector units solve.
     acc4 vsc[kTY], vsn[kTY];
     m2a(0, vsc);
-    if (NAB == 1) fetch_vi(zn, en);
 #pragma unroll
     for (int a = 0; a < NAB; ++a) {
-      if (a + 1 < NAB) {
-        m2a(a + 1 < NAB ? a + 1 : 0, vsn);
-        if (a + 2 == NAB) fetch_vi(zn, en);
-      }
+      if (a + 1 < NAB) m2a(a + 1 < NAB ? a + 1 : 0, vsn);
       // solve, blocks in descending order: block b's y face is block b-1's
       // value from the previous step, read before it is overwritten
       // y inflow of row 0 of a lower row group: the row group above wrote it
@@ -455,8 +494,10 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
               yr);
 #pragma unroll
       for (int b = kTY - 1; b >= 0; --b) {
+        // x inflow of lane (0, g): row (g, r) of block b sits in lane 4b + r
         T xin[4];
-        load4(xs_lds + a * 64 + 16 * g + 4 * b, xin);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) xin[r] = from_lane(xsc[a], 4 * b + r);
         const T cz = ldexpT((T)0x1p-9, (int)((lut_z >> sh[b]) & 15));
         const T cy = ldexpT((T)0x1p-10, (int)((lut_y >> sh[b]) & 15));
         const int xe = (int)((lut_x >> sh[b]) & 15) - 8;
@@ -573,6 +614,22 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
     for (int b = 0; b < kTY; ++b) {
       zs[b] = zn[b];
       es[b] = en[b];
+    }
+  };
+  if constexpr (PR::kPingPong) {
+    for (int t = 0; t < nsteps; t += 2) {
+      step(t, vbA, vbB, xbA, xbB);
+      if (t + 1 < nsteps) step(t + 1, vbB, vbA, xbB, xbA);
+    }
+  } else {
+    for (int t = 0; t < nsteps; ++t) {
+#pragma unroll
+      for (int a = 0; a < NAB; ++a) xbA[a] = xbB[a];
+#pragma unroll
+      for (int b = 0; b < kTY; ++b)
+#pragma unroll
+        for (int s = 0; s < KA; ++s) vbA[b][s] = vbB[b][s];
+      step(t, vbA, vbB, xbA, xbB);
     }
   }
 }
@@ -794,7 +851,7 @@ size_t mfma_lds_bytes(int precision, int nac, int nwa, int ny) {
   const size_t nab = nac / 16;
   const size_t red = nwa > 1 ? 2 * (size_t)ny * kTY * nwa * 64 * 4 * t : 0;
   const size_t ring = ny > 1 ? 2 * (size_t)(ny - 1) * nwa * nab * 64 * 4 * t : 0;
-  return red + ring + (size_t)ny * nwa * nab * 64 * t;
+  return red + ring;
 }
 
 int mfma_tile_x() { return kTX; }

@@ -77,9 +77,14 @@ int default_mfma_ny(int precision, int nwa, int ncell_y) {
   return 1;
 }
 
-int launch_mfma_epw(int preferred, int ne, long long waves1) {
+// Energy groups per workgroup for one launch of waves1 waves per group: the
+// preferred count, lowered while the launch would have fewer waves than the
+// target (16 per SIMD for FP64, 8 for the mixed forms, whose longer streams
+// pay off more: measured at 64^3).
+int launch_mfma_epw(int preferred, int ne, long long waves1, int precision) {
+  const long long target = precision == MSW_PREC_FP64 ? 4096 : 2048;
   int e = preferred < 1 ? 1 : preferred > ne ? ne : preferred;
-  while (e > 1 && waves1 * ((ne + e - 1) / e) < 4096) --e;
+  while (e > 1 && waves1 * ((ne + e - 1) / e) < target) --e;
   return e;
 }
 
@@ -728,12 +733,13 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
                                      (p->nwa > 1 ? (size_t)p->nwa * d->nm * 64 : 0));
   auto launches = build_schedule(p->ntx, p->nty, 0);
   // energy groups per workgroup, per launch: streamed while the launch
-  // keeps >= 4 waves per SIMD (1024 SIMDs), or as the caller fixed it
+  // keeps its target wave count, or as the caller fixed it
   for (auto& l : launches) {
-    const long long w1 = (long long)l.size() * d->nu * p->nwa;
+    const long long w1 = (long long)l.size() * d->nu * p->nwa * p->ny;
     p->launch_epw.push_back(opt->groups_per_workgroup > 0
                                 ? std::min(opt->groups_per_workgroup, d->ne)
-                                : launch_mfma_epw(default_mfma_epw(d->ncell_z, d->ne), d->ne, w1));
+                                : launch_mfma_epw(default_mfma_epw(d->ncell_z, d->ne), d->ne, w1,
+                                                  precision));
   }
   std::vector<TileItem> flat;
   for (auto& l : launches) {
