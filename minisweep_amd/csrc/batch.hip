@@ -122,6 +122,8 @@ int make_geom(const MswDims& d, int nac, Geom* g, int ny = 0, int precision = MS
   g->nwa = (d.na + nac - 1) / nac;
   if (g->nwa > 4) return set_error(MSW_ERR_UNSUPPORTED, "na above 4 waves of angles");
   g->nap = g->nwa * nac;
+  if (!mfma_face_offsets_fit(d, g->nap))
+    return set_error(MSW_ERR_UNSUPPORTED, "face planes too large for 32-bit offsets");
   g->ntx = (d.ncell_x + mfma_tile_x() - 1) / mfma_tile_x();
   // requested row groups, or fewer where that workgroup shape is not built
   g->ny = ny > 0 ? ny : default_mfma_ny(precision, g->nwa, d.ncell_y);

@@ -46,6 +46,9 @@ int mfma_fill_rows(const SweepParams& sp);
 size_t mfma_face_elem_bytes(int precision);
 // MFMA tile: mfma_tile_x() cells along x, ny row groups of mfma_tile_rows()
 // rows along y.
+// The sweep kernel addresses one (octant, unknown) face plane with 32-bit
+// offsets: max(ncell_x, ncell_y) * ne * ncell_z * na_pad < 2^31.
+bool mfma_face_offsets_fit(const MswDims& d, int na_pad);
 int mfma_tile_x();
 int mfma_tile_rows();
 

@@ -265,16 +265,21 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
   int yb[kTY], cst[kTY];
   const size_t Se = (size_t)ny * nx * nu * NM;  // state stride of a group
   const size_t Sz = (size_t)p.ne * Se;          // state stride of a z plane
-  // state offset of the lane's cells at z = 0 of group ie0: a per-lane x part
-  // plus a per-block y part that is the same for all lanes
-  const size_t cxo = (size_t)ie0 * Se + ((size_t)x * nu + iu) * NM;
-  size_t coff[kTY];
+  // State offset of each block's cell, kept incrementally (no multiplies in
+  // the z loop): it starts at z = 0 of group ie0 in the octant's frame and
+  // moves by dS per step, by dW where the stream enters the next group, and
+  // stays where it is before and after the stream (clamped, never used).
+  const size_t cxo = (size_t)ie0 * Se + ((size_t)x * nu + iu) * NM +
+                     (dz > 0 ? (size_t)0 : (size_t)(nz - 1) * Sz);
+  const long long dS = dz > 0 ? (long long)Sz : -(long long)Sz;
+  const long long dW = (long long)Se - (long long)(nz - 1) * dS;
+  size_t sc[kTY];  // this step's offsets
 #pragma unroll
   for (int b = 0; b < kTY; ++b) {
     const int r = kTY * wyi + b;
     const int bc = r < wy ? r : wy - 1;
     yb[b] = dy > 0 ? gy * kTYT + bc : gy * kTYT + wy - 1 - bc;
-    coff[b] = cxo + (size_t)yb[b] * nx * nu * NM;
+    sc[b] = cxo + (size_t)yb[b] * nx * nu * NM;
     cst[b] = x + 3 * yb[b] + 2;
   }
   const int zb = dz > 0 ? -1 : nz;
@@ -286,13 +291,16 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
   T* yring = reinterpret_cast<T*>(red + (NWA > 1 ? 2 * NY * kTY * NWA * 64 * 4 : 0));
 
   // ---- face planes of this (octant, unknown): [y or x][group][z][angle],
-  // z in the octant's frame so that a stream position maps to one offset --
+  // z in the octant's frame so that a stream position maps to one offset.
+  // Offsets within one (octant, unknown) plane fit 32 bits (the plan checks
+  // max(nx, ny) * ne * nz * na_pad < 2^31). --
   const size_t nze = (size_t)p.ne * nz;  // stream length of all groups
+  const int fplane = (int)nze * nap;     // one y (x face) or x (y face) row
   const size_t face_ou = (size_t)oct * nu + iu;
   T* fyz = reinterpret_cast<T*>(p.fyz) + (face_ou * ny * nze + (size_t)ie0 * nz) * nap + ia0 +
            4 * g;
   T* fxz = reinterpret_cast<T*>(p.fxz) + (face_ou * nx * nze + (size_t)ie0 * nz) * nap + ia0 +
-           4 * g;
+           4 * g + x * fplane;  // this lane's column
 
   // ---- per-octant operands, in registers for the whole launch --------------
   const int moff = MSTEP * g;
@@ -360,34 +368,44 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
     zs[b] = -j - b - kTY * wyi;
     es[b] = 0;
   }
-  // state offset of (group offset e, octant-frame z) in a block, clamped
-  auto soff = [&](int b, int e, int zq) {
-    const int zc = zq < 0 ? 0 : zq;
-    const int ec = e < nge ? e : nge - 1;
-    const int zg = dz > 0 ? zc : nz - 1 - zc;
-    return coff[b] + (size_t)ec * Se + (size_t)zg * Sz;
-  };
-  // face-plane offset of stream position sp, clamped
-  auto foff = [&](int sp) {
-    const int sc = sp < 0 ? 0 : sp >= ns ? ns - 1 : sp;
-    return (size_t)sc * nap;
-  };
   int zn[kTY], en[kTY];  // next step's stream position per block
+  size_t sn[kTY];        // next step's state offsets
   auto advance = [&]() {
 #pragma unroll
     for (int b = 0; b < kTY; ++b) {
       const bool wrap = zs[b] + 1 == nz;
       zn[b] = wrap ? 0 : zs[b] + 1;
       en[b] = wrap ? es[b] + 1 : es[b];
+      const bool move = zs[b] >= 0 && en[b] < nge;
+      sn[b] = sc[b] + (size_t)(move ? (wrap ? dW : dS) : 0ll);
     }
   };
+  // Face-plane offsets.  Stores of block b's outflow go to stream position
+  // t - j - b - 4 wyi, valid whenever they happen: fo0 - b * nap with fo0 the
+  // position t - j - 4 wyi times nap.  The inflow loads of the next step use
+  // clamped positions, kept as (position, offset) pairs.
+  int fo0 = (-j - kTY * wyi) * nap;
+  const int bb = j >> 2;  // x inflow: lane (j, g) fetches block j/4's value
+  int pxs = -bb - kTY * wyi, oxs = 0;  // x inflow position of the step fetched
+  int pyi = -j, oyi = 0;               // y inflow position (row group 0)
+  auto bump = [&](int& pos, int& off) {  // position + 1, offset clamped to [0, ns)
+    ++pos;
+    off += pos >= 1 && pos < ns ? nap : 0;
+  };
+  const T* xs_src;
+  {
+    const int r = kTY * wyi + bb;
+    const int bc = r < wy ? r : wy - 1;
+    const int y = dy > 0 ? gy * kTYT + bc : gy * kTYT + wy - 1 - bc;
+    xs_src = fyz + y * fplane + (j & 3);
+  }
   // the input state: FP64, or its FP32 copy in the BF16 form
   typedef typename std::conditional<PR::kPacked, float, double>::type VI;
   const VI* vsrc = PR::kPacked ? (const VI*)p.vi32 : (const VI*)p.vi;
-  auto fetch_vi = [&](const int (&ze)[kTY], const int (&ee)[kTY], T (&vnx)[kTY][KA]) {
+  auto fetch_vi = [&](const size_t (&so)[kTY], T (&vnx)[kTY][KA]) {
 #pragma unroll
     for (int b = 0; b < kTY; ++b) {
-      const VI* src = vsrc + soff(b, ee[b], ze[b]) + moff;
+      const VI* src = vsrc + so[b] + moff;
       if constexpr (KA == 4) {
         VI v[4] = {0, 0, 0, 0};
         if (!mpad) load4(src, v);
@@ -403,39 +421,34 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
   // no sum to read; it loads 16 bytes of the octant matrices instead (cached,
   // never used), so the load is still issued on every path.
   const V* vrd = first ? (const V*)p.at : kF64 ? (const V*)p.vo : (const V*)p.vo32;
-  auto fetch_vo = [&](const int (&ze)[kTY], const int (&ee)[kTY]) {
+  auto fetch_vo = [&](const size_t (&so)[kTY]) {
 #pragma unroll
     for (int b = 0; b < kTY; ++b)
       if (b % NWA == wa && 4 * g < NM)
-        load4(vrd + (first ? 0 : soff(b, ee[b], ze[b])) + 4 * g, von[b / NWA]);
+        load4(vrd + (first ? 0 : so[b]) + 4 * g, von[b / NWA]);
   };
-  auto fetch_xs = [&](int t, T (&xsn)[NAB]) {
-    const int bb = j >> 2;
-    const int r = kTY * wyi + bb;
-    const int bc = r < wy ? r : wy - 1;
-    const int y = dy > 0 ? gy * kTYT + bc : gy * kTYT + wy - 1 - bc;
-    const T* src = fyz + (size_t)y * nze * nap + foff(t - bb - kTY * wyi) + (j & 3);
+  auto fetch_xs = [&](T (&xsn)[NAB]) {
 #pragma unroll
-    for (int a = 0; a < NAB; ++a) xsn[a] = src[16 * a];
+    for (int a = 0; a < NAB; ++a) xsn[a] = xs_src[oxs + 16 * a];
   };
-  auto fetch_yin = [&](int t) {  // row group 0 only
-    const T* src = fxz + (size_t)x * nze * nap + foff(t - j);
+  auto fetch_yin = [&]() {  // row group 0 only
 #pragma unroll
-    for (int a = 0; a < NAB; ++a) load4(src + 16 * a, yinn[a]);
+    for (int a = 0; a < NAB; ++a) load4(fxz + oyi + 16 * a, yinn[a]);
   };
 
-  fetch_xs(0, PR::kPingPong ? xbA : xbB);
-  fetch_vi(zs, es, PR::kPingPong ? vbA : vbB);
-  if (wyi == 0) fetch_yin(0);
-  fetch_vo(zs, es);
+  fetch_xs(PR::kPingPong ? xbA : xbB);
+  fetch_vi(sc, PR::kPingPong ? vbA : vbB);
+  if (wyi == 0) fetch_yin();
+  fetch_vo(sc);
 
   // One step: this step's operands are vin and xsc, the next step's are
   // fetched into vnx and xsn first thing.
   auto step = [&](const int t, const T (&vin)[kTY][KA], T (&vnx)[kTY][KA], const T (&xsc)[NAB],
                   T (&xsn)[NAB]) {
     advance();
-    fetch_vi(zn, en, vnx);
-    fetch_xs(t + 1, xsn);
+    fetch_vi(sn, vnx);
+    bump(pxs, oxs);
+    fetch_xs(xsn);
 
     // ---- per cell block: validity, z, spatial phase; z inflow on entry -------
     // A lane whose cell starts a group (z = 0) takes the domain-boundary z
@@ -444,14 +457,12 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
     // as their inflow in this step.
     bool val[kTY], ent[kTY];
     int zgs[kTY], sh[kTY];
-    size_t so[kTY];
 #pragma unroll
     for (int b = 0; b < kTY; ++b) {
       val[b] = x_ok && kTY * wyi + b < wy && zs[b] >= 0 && es[b] < nge;
       ent[b] = zs[b] == 0 && es[b] < nge;
       const int zc = zs[b] < 0 ? 0 : zs[b];
       zgs[b] = dz > 0 ? zc : nz - 1 - zc;
-      so[b] = soff(b, es[b], zs[b]);
       sh[b] = ((cst[b] + 7 * zgs[b]) & 3) * 4;
     }
 
@@ -538,7 +549,10 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
         for (int b = 0; b < kTY; ++b) vsc[b] = vsn[b];
       }
     }
-    if (wyi == 0) fetch_yin(t + 1);
+    if (wyi == 0) {
+      bump(pyi, oyi);
+      fetch_yin();
+    }
     // the last row of this row group is the next row group's y inflow
     if (NY > 1 && wyi < NY - 1) {
 #pragma unroll
@@ -553,7 +567,7 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
 #pragma unroll
       for (int b = 0; b < kTY; ++b)
         if (val[b]) {
-          T* dst = fyz + (size_t)yb[b] * nze * nap + foff(t - j - b - kTY * wyi);
+          T* dst = fyz + (yb[b] * fplane + fo0 - b * nap);
 #pragma unroll
           for (int a = 0; a < NAB; ++a) store4(dst + 16 * a, F[a][b]);
         }
@@ -562,7 +576,7 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
 #pragma unroll
       for (int b = 0; b < kTY; ++b)
         if (kTY * wyi + b == wy - 1 && val[b]) {
-          T* dst = fxz + (size_t)x * nze * nap + foff(t - j - b - kTY * wyi);
+          T* dst = fxz + (fo0 - b * nap);
 #pragma unroll
           for (int a = 0; a < NAB; ++a) store4(dst + 16 * a, F[a][b]);
         }
@@ -602,19 +616,21 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
       }
       if (val[b] && 4 * g < NM) {
         if (kF64 || last) {
-          store4(p.vo + so[b] + 4 * g, sum);
+          store4(p.vo + sc[b] + 4 * g, sum);
         } else {
           float f[4] = {(float)sum[0], (float)sum[1], (float)sum[2], (float)sum[3]};
-          store4(p.vo32 + so[b] + 4 * g, f);
+          store4(p.vo32 + sc[b] + 4 * g, f);
         }
       }
     }
-    fetch_vo(zn, en);
+    fetch_vo(sn);
 #pragma unroll
     for (int b = 0; b < kTY; ++b) {
       zs[b] = zn[b];
       es[b] = en[b];
+      sc[b] = sn[b];
     }
+    fo0 += nap;
   };
   if constexpr (PR::kPingPong) {
     for (int t = 0; t < nsteps; t += 2) {
@@ -852,6 +868,11 @@ size_t mfma_lds_bytes(int precision, int nac, int nwa, int ny) {
   const size_t red = nwa > 1 ? 2 * (size_t)ny * kTY * nwa * 64 * 4 * t : 0;
   const size_t ring = ny > 1 ? 2 * (size_t)(ny - 1) * nwa * nab * 64 * 4 * t : 0;
   return red + ring;
+}
+
+bool mfma_face_offsets_fit(const MswDims& d, int na_pad) {
+  const long long ext = d.ncell_x > d.ncell_y ? d.ncell_x : d.ncell_y;
+  return ext * d.ne * d.ncell_z * na_pad < (1ll << 31);
 }
 
 int mfma_tile_x() { return kTX; }

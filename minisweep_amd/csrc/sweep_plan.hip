@@ -705,9 +705,13 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
                    "", d->na);
   }
   p->na_pad = p->nwa * p->nac;
+  if (p->engine == MSW_ENGINE_MFMA && !mfma_face_offsets_fit(*d, p->na_pad)) {
+    delete p;
+    return set_error(MSW_ERR_UNSUPPORTED, "face planes too large for 32-bit offsets");
+  }
   if (opt->groups_per_workgroup < 0) {
     delete p;
-    return set_err(MSW_ERR_INVALID, "groups_per_workgroup must be >= 0%s%ld");
+    return set_error(MSW_ERR_INVALID, "groups_per_workgroup must be >= 0");
   }
   // tile of one wave: 8x8 cells (VALU engine), 16x4 (MFMA engine)
   const int tx = p->engine == MSW_ENGINE_MFMA ? mfma_tile_x() : kTile;
