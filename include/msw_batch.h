@@ -107,6 +107,14 @@ int msw_server_create(const MswServerOptions* opt, MswServer** server);
 int msw_server_submit(MswServer* server, const MswDims* d, long long tag,
                       const double* vi_host);
 
+/* Serve with the kernel configuration an autotuned batch of the same
+ * precision settled on (angles per wave per (nm, na) class, tile rows, stream
+ * length, schedule heuristic): tune once on representative problems, then
+ * serve.  Applies to requests admitted after the call.  A request's result is
+ * then the same bits as its msw_plan_sweep with those angles per wave and
+ * tile rows. */
+int msw_server_tune_from_batch(MswServer* server, const MswBatch* batch);
+
 /* Completed requests (up to max); returns the count or a negated error. */
 int msw_server_poll(MswServer* server, MswResult* out, int max);
 

@@ -118,6 +118,7 @@ def lib():
         L.msw_server_poll.argtypes = [vp, ctypes.POINTER(MswResult), ctypes.c_int]
         L.msw_server_fetch.argtypes = [vp, ctypes.c_longlong, ctypes.POINTER(ctypes.c_double)]
         L.msw_server_drain.argtypes = [vp, ctypes.c_double]
+        L.msw_server_tune_from_batch.argtypes = [vp, vp]
         L.msw_server_now.argtypes = [vp]
         L.msw_server_now.restype = ctypes.c_double
         L.msw_server_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
@@ -344,6 +345,10 @@ class Server:
 
     def drain(self, timeout_s: float = 0.0):
         _check(lib().msw_server_drain(self._h, timeout_s), "msw_server_drain")
+
+    def tune_from(self, batch: "Batch"):
+        """Serve with an autotuned batch's kernel configuration."""
+        _check(lib().msw_server_tune_from_batch(self._h, batch._h), "msw_server_tune_from_batch")
 
     def now(self) -> float:
         return float(lib().msw_server_now(self._h))

@@ -804,6 +804,11 @@ struct MswServer {
   double t0 = 0;
   long long ndone = 0, nlaunch = 0, nticks = 0;
   int in_flight = 0;  // requests admitted and not yet reported
+  // kernel configuration of admitted problems (msw_server_tune_from_batch):
+  // angles per wave per (nm, na) class (absent: the plan default), row groups
+  // (0: the plan default), z-stream steps per workgroup, schedule heuristic
+  std::map<std::pair<int, int>, int> class_nac;
+  int row_groups = 0, stream_target = 128, schedule_kind = 0;
 };
 
 namespace {
@@ -883,9 +888,16 @@ void server_loop(MswServer* s) {
         Request& q = s->queue.front();
         Active a;
         a.req = q;
-        int rc = make_geom(q.d, default_mfma_nac(s->opt.precision, q.d.na), &a.g, 0,
-                           s->opt.precision);
-        if (!rc) rc = get_schedule(s->scheds, a.g.ntx, a.g.nty, 0, &a.s);
+        const auto cn = s->class_nac.find(std::make_pair(q.d.nm, q.d.na));
+        int rc = make_geom(q.d,
+                           cn != s->class_nac.end() ? cn->second
+                                                    : default_mfma_nac(s->opt.precision, q.d.na),
+                           &a.g, s->row_groups, s->opt.precision);
+        if (!rc) {
+          const int nz = a.g.d.ncell_z;
+          a.g.epw = std::max(1, std::min(a.g.d.ne, (s->stream_target + nz / 2) / nz));
+          rc = get_schedule(s->scheds, a.g.ntx, a.g.nty, s->schedule_kind, &a.s);
+        }
         if (!rc) rc = server_matrices(s, a.g, &a.at, &a.mf);
         if (rc) {
           a.slot = s->free_slots.back();
@@ -1012,6 +1024,20 @@ void msw_server_default_options(MswServerOptions* o) {
   o->hbm_budget_bytes = (size_t)32 << 30;
   o->max_active = 512;
   o->keep_outputs = 0;
+}
+
+int msw_server_tune_from_batch(MswServer* s, const MswBatch* b) {
+  if (!s || !b) return set_error(MSW_ERR_INVALID, "null server or batch");
+  if (b->opt.precision != s->opt.precision)
+    return set_error(MSW_ERR_INVALID, "batch and server precisions differ");
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->class_nac.clear();
+  for (const auto& kv : b->class_nac)
+    if (kv.second) s->class_nac[kv.first] = kv.second;
+  s->row_groups = b->row_groups;
+  s->stream_target = b->stream_target;
+  s->schedule_kind = b->opt.schedule_kind;
+  return MSW_OK;
 }
 
 int msw_server_create(const MswServerOptions* opt, MswServer** out) {

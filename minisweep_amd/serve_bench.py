@@ -13,6 +13,11 @@ Latency = completion - submission on the server clock; throughput = requests
 submitted at once) is measured for both.  Every result is checked against the
 closed-form normsq.
 
+Before serving, the served form is tuned once: a batch of --tune-problems
+problems from the same generator (seed 0, the batched workload) is autotuned
+and the server takes its kernel configuration (msw_server_tune_from_batch).
+--no-tune serves with the plan defaults.
+
     python -m minisweep_amd.serve_bench [--rates 50,200,800] [--requests 10000]
 
 Prints one JSON line per measurement and a summary line.
@@ -28,9 +33,33 @@ import time
 import numpy as np
 
 
-def run_stream(msw, server_kw, shapes, arrivals, expect, label, progress):
+def tuned_batch(msw, precision, nprob):
+    """An autotuned batch of nprob problems of the batched workload (seed 0)."""
+    import torch
+    from minisweep_amd.workloads import uneven_problems
+    dev = torch.device("cuda:0")
+    dims = [msw.dims(*s) for s in uneven_problems(nprob, 0)]
+    b = msw.Batch(dims, precision=precision)
+    mats, bufs = {}, []
+    for i, d in enumerate(dims):
+        if (d.nm, d.na) not in mats:
+            a, m = msw.init_matrices(d.nm, d.na)
+            mats[(d.nm, d.na)] = (torch.from_numpy(a).to(dev), torch.from_numpy(m).to(dev))
+        vi = torch.empty(msw.size_state(d), dtype=torch.float64, device=dev)
+        msw.init_state(d, vi)
+        vo = torch.empty_like(vi)
+        bufs.append((vi, vo))
+        b.set_problem(i, vi, vo, *mats[(d.nm, d.na)])
+    b.autotune(reps=2)
+    torch.cuda.synchronize()
+    return b, bufs
+
+
+def run_stream(msw, server_kw, shapes, arrivals, expect, label, progress, tune=None):
     """Submit requests at the arrival times (None: all at once); returns stats."""
     srv = msw.Server(**server_kw)
+    if tune is not None:
+        srv.tune_from(tune)
     # warm the per-shape caches (schedules, matrices) outside the measurement
     seen = set()
     for i, s in enumerate(shapes):
@@ -97,6 +126,8 @@ def main():
     ap.add_argument("--max-active", type=int, default=256)
     ap.add_argument("--precision", choices=["fp32", "bf16", "fp64"], default="fp32")
     ap.add_argument("--skip-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tune-problems", type=int, default=256)
     args = ap.parse_args()
 
     import minisweep_amd as msw
@@ -117,11 +148,18 @@ def main():
         print(json.dumps(rec), flush=True)
         out.append(rec)
 
+    tune, bufs, tuned_desc = None, None, None
+    if not args.no_tune:
+        progress(f"autotuning the served form on {args.tune_problems} problems")
+        tune, bufs = tuned_batch(msw, prec, args.tune_problems)
+        tuned_desc = tune.describe()
+        tuned_desc.pop("tuning", None)
+
     nsat = args.saturation_requests
     shapes = uneven_problems(nsat, args.seed)
     expect = [float(closed_form_normsq(s[0], s[1], s[2], s[3], s[4], s[6])) for s in shapes]
     sat = run_stream(msw, served, shapes, None, expect, f"served {args.precision} saturation",
-                     progress)
+                     progress, tune)
     emit(sat)
     sat_seq = None
     if not args.skip_baseline:
@@ -135,7 +173,7 @@ def main():
                   for s in shapes]
         arr = poisson_arrivals(args.requests, rate, args.seed)
         rec = run_stream(msw, served, shapes, arr, expect,
-                         f"served {args.precision} at {rate:g} req/s", progress)
+                         f"served {args.precision} at {rate:g} req/s", progress, tune)
         rec["rate"] = rate
         emit(rec)
         if not args.skip_baseline:
@@ -151,7 +189,10 @@ def main():
         "speedup": (round(sat["problems_per_s"] / sat_seq["problems_per_s"], 2)
                     if sat_seq else None),
         "all_correct": all(r["wrong_results"] == 0 for r in out),
+        "served_config": tuned_desc,
     }
+    if tune is not None:
+        tune.close()
     print(json.dumps(summary), flush=True)
 
 

@@ -205,3 +205,42 @@ def test_server_admission_by_footprint(cuda):
     r = srv.poll()
     assert len(r) == 1 and r[0]["tag"] == 999 and r[0]["status"] == 4  # MSW_ERR_NOMEM
     srv.close()
+
+
+def test_server_tuned_from_batch(cuda):
+    """A server that takes a batch's kernel configuration gives each request
+    the bits of its single sweep with that configuration."""
+    import torch
+    print("case server tuned from batch", flush=True)
+    s = SHAPES[0]  # nm 16, na 64
+    d = msw.dims(*s)
+    b = msw.Batch([d], precision=msw.PREC_MIXED_FP32)
+    b.set_class_nac(16, 64, 32)  # not the mixed default (64)
+    rng = np.random.default_rng(5)
+    base = oracle.init_state(oracle.dims(*s))
+    vi0 = base * (1.0 + 0.25 * rng.uniform(-1, 1, base.size))
+    srv = msw.Server(precision=msw.PREC_MIXED_FP32, hbm_budget_bytes=1 << 30, max_active=4,
+                     keep_outputs=True)
+    srv.tune_from(b)
+    srv.submit(d, 1, vi0)
+    srv.drain(60)
+    res = srv.poll()
+    assert len(res) == 1 and res[0]["status"] == 0
+    got = srv.fetch(1, d)
+    srv.close()
+    a, m = msw.init_matrices(d.nm, d.na)
+    vi = torch.from_numpy(vi0).to(cuda)
+    vo = torch.full_like(vi, float("nan"))
+    plan = msw.Plan(d, msw.PREC_MIXED_FP32, engine=msw.ENGINE_MFMA, angles_per_wave=32)
+    plan.sweep(vi, vo, torch.from_numpy(a).to(cuda), torch.from_numpy(m).to(cuda))
+    torch.cuda.synchronize()
+    plan.close()
+    np.testing.assert_array_equal(got, vo.cpu().numpy())
+    # a batch of another precision is refused
+    b64 = msw.Batch([d], precision=msw.PREC_FP64)
+    srv = msw.Server(precision=msw.PREC_MIXED_FP32, hbm_budget_bytes=1 << 30, max_active=4)
+    with pytest.raises(RuntimeError):
+        srv.tune_from(b64)
+    srv.close()
+    b.close()
+    b64.close()
