@@ -62,8 +62,9 @@ int msw_batch_run(MswBatch* batch, void* stream);
 
 /* Autotune: time each candidate (angles per wave of every (nm, na) class in
  * turn, then the tile height in row groups, the energy groups streamed per
- * workgroup, the schedule heuristic and staggering) over reps runs and keep
- * the fastest.  Runs the batch many times; the problems' vo are overwritten. */
+ * workgroup, the classes' angles per wave once more, the schedule heuristic
+ * and staggering) over reps runs and keep the fastest.  Runs the batch many
+ * times; the problems' vo are overwritten. */
 int msw_batch_autotune(MswBatch* batch, void* stream, int reps);
 
 /* Angles per wave used for problems with this (nm, na) (0 = default). */

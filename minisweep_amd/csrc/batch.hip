@@ -281,6 +281,9 @@ struct BatchProb {
 
 }  // namespace
 
+// Streams the tick chains of different kernel variants run on concurrently.
+constexpr int kChainStreams = 4;
+
 struct MswBatch {
   MswBatchOptions opt{};
   std::vector<MswDims> dims;
@@ -298,6 +301,10 @@ struct MswBatch {
   int stream_target = 128;  // z-stream steps per workgroup (energy groups streamed)
   int row_groups = 0;       // row groups of 4 rows per tile (0: the plan default)
   std::string log;
+  // the kernel variants' tick chains run on their own streams, forked from
+  // and joined back to the run's stream (also inside the captured graph)
+  hipStream_t chain_st[kChainStreams] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[kChainStreams] = {};
 };
 
 namespace {
@@ -378,14 +385,44 @@ int batch_enqueue(MswBatch* b, hipStream_t st) {
   nk += enqueue_fill(prec, fill, b->d_params, st);
   int nt = 0;
   for (auto& p : b->probs) nt = std::max(nt, p.start_tick + p.s->nlaunch());
-  for (int t = 0; t < nt; ++t) {
-    std::vector<TickWork> work;
-    for (size_t i = 0; i < b->probs.size(); ++i) {
-      const BatchProb& p = b->probs[i];
-      const int l = t - p.start_tick;
-      if (l >= 0 && l < p.s->nlaunch()) work.push_back(TickWork{(int)i, &p.g, p.s, l});
+  // One chain of ticks per kernel variant: problems of different variants
+  // share no launch, so the chains run concurrently on their own streams
+  // (round-robin over kChainStreams), forked after the fill and joined at the
+  // end.  Within a chain, tick t+1 follows tick t.
+  std::map<std::tuple<int, int, int, int>, std::vector<int>> chains;
+  for (size_t i = 0; i < b->probs.size(); ++i) chains[b->probs[i].g.variant()].push_back((int)i);
+  const int nch = std::min<int>((int)chains.size(), kChainStreams);
+  if (nch > 1) {
+    for (int c = 0; c < nch; ++c) {
+      if (!b->chain_st[c])
+        HIP_OK(hipStreamCreateWithFlags(&b->chain_st[c], hipStreamNonBlocking), "chain stream");
+      if (!b->join_ev[c])
+        HIP_OK(hipEventCreateWithFlags(&b->join_ev[c], hipEventDisableTiming), "chain event");
     }
-    if (!work.empty()) nk += enqueue_tick(prec, work, b->d_params, st);
+    if (!b->fork_ev)
+      HIP_OK(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming), "chain event");
+    HIP_OK(hipEventRecord(b->fork_ev, st), "fork");
+    for (int c = 0; c < nch; ++c) HIP_OK(hipStreamWaitEvent(b->chain_st[c], b->fork_ev, 0), "fork");
+  }
+  int ci = 0;
+  for (auto& kv : chains) {
+    hipStream_t cs = nch > 1 ? b->chain_st[ci % nch] : st;
+    ++ci;
+    for (int t = 0; t < nt; ++t) {
+      std::vector<TickWork> work;
+      for (int i : kv.second) {
+        const BatchProb& p = b->probs[i];
+        const int l = t - p.start_tick;
+        if (l >= 0 && l < p.s->nlaunch()) work.push_back(TickWork{i, &p.g, p.s, l});
+      }
+      if (!work.empty()) nk += enqueue_tick(prec, work, b->d_params, cs);
+    }
+  }
+  if (nch > 1) {
+    for (int c = 0; c < nch; ++c) {
+      HIP_OK(hipEventRecord(b->join_ev[c], b->chain_st[c]), "join");
+      HIP_OK(hipStreamWaitEvent(st, b->join_ev[c], 0), "join");
+    }
   }
   b->nticks = nt;
   b->nkernels = nk;
@@ -577,27 +614,33 @@ int msw_batch_autotune(MswBatch* b, void* stream, int reps) {
   for (auto& d : b->dims)
     if (std::find(classes.begin(), classes.end(), std::make_pair(d.nm, d.na)) == classes.end())
       classes.emplace_back(d.nm, d.na);
-  for (auto& c : classes) {
-    const int cur = nac_for(b, MswDims{1, 1, 1, 1, c.first, c.second, 1});
-    int keep = cur;
-    for (int nac : {16, 32, 64}) {
-      if (nac == cur || (c.second + nac - 1) / nac > 4 || nac > 16 * ((c.second + 15) / 16))
-        continue;
-      b->class_nac[c] = nac;
-      b->built = false;
-      double ms = 0;
-      rc = batch_time(b, st, reps, &ms);
-      if (rc) return rc;
-      note("nm" + std::to_string(c.first) + " na" + std::to_string(c.second) + " nac" +
-               std::to_string(nac), ms);
-      if (ms < best) {
-        best = ms;
-        keep = nac;
+  // angles per wave of each class, one class at a time (run again after the
+  // tile height and stream length have moved)
+  auto class_pass = [&]() -> int {
+    for (auto& c : classes) {
+      const int cur = nac_for(b, MswDims{1, 1, 1, 1, c.first, c.second, 1});
+      int keep = cur;
+      for (int nac : {16, 32, 64}) {
+        if (nac == cur || (c.second + nac - 1) / nac > 4 || nac > 16 * ((c.second + 15) / 16))
+          continue;
+        b->class_nac[c] = nac;
+        b->built = false;
+        double ms = 0;
+        const int e = batch_time(b, st, reps, &ms);
+        if (e) return e;
+        note("nm" + std::to_string(c.first) + " na" + std::to_string(c.second) + " nac" +
+                 std::to_string(nac), ms);
+        if (ms < best) {
+          best = ms;
+          keep = nac;
+        }
       }
+      b->class_nac[c] = keep;
+      b->built = false;
     }
-    b->class_nac[c] = keep;
-    b->built = false;
-  }
+    return MSW_OK;
+  };
+  if ((rc = class_pass())) return rc;
   for (int rg : {1, 2, 4}) {
     if (rg == b->row_groups) continue;
     const int save = b->row_groups;
@@ -613,7 +656,8 @@ int msw_batch_autotune(MswBatch* b, void* stream, int reps) {
       b->row_groups = save;
     b->built = false;
   }
-  for (int target : {64, 256}) {
+  for (int target : {32, 64, 256}) {
+    if (target == b->stream_target) continue;
     const int save = b->stream_target;
     b->stream_target = target;
     b->built = false;
@@ -627,6 +671,7 @@ int msw_batch_autotune(MswBatch* b, void* stream, int reps) {
       b->stream_target = save;
     b->built = false;
   }
+  if ((rc = class_pass())) return rc;
   for (int kind : {0, 1})
     for (int stag : {0, 1}) {
       if (kind == b->opt.schedule_kind && stag == b->opt.stagger) continue;
@@ -684,6 +729,11 @@ int msw_batch_destroy(MswBatch* b) {
   batch_release(b);
   free_schedules(b->scheds);
   if (b->cap) (void)hipStreamDestroy(b->cap);
+  for (int c = 0; c < kChainStreams; ++c) {
+    if (b->chain_st[c]) (void)hipStreamDestroy(b->chain_st[c]);
+    if (b->join_ev[c]) (void)hipEventDestroy(b->join_ev[c]);
+  }
+  if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
   delete b;
   return MSW_OK;
 }
