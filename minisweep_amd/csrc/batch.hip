@@ -231,6 +231,26 @@ int enqueue_tick(int precision, std::vector<TickWork>& work, const SweepParams* 
 
 // Boundary fill of the face planes of several problems (before their first
 // tile launch).  Returns the number of launches.
+// One tick with each kernel variant's launches on a stream of its own
+// (round-robin over ncs streams), forked from st and joined back to it.
+int enqueue_tick_forked(int precision, std::vector<TickWork>& work, const SweepParams* d_params,
+                        hipStream_t st, hipStream_t* cs, hipEvent_t fork, hipEvent_t* join,
+                        int ncs) {
+  std::map<std::tuple<int, int, int, int>, std::vector<TickWork>> by_variant;
+  for (const TickWork& w : work) by_variant[w.g->variant()].push_back(w);
+  const int n = std::min<int>((int)by_variant.size(), ncs);
+  if (n < 2) return enqueue_tick(precision, work, d_params, st);
+  (void)hipEventRecord(fork, st);
+  for (int c = 0; c < n; ++c) (void)hipStreamWaitEvent(cs[c], fork, 0);
+  int nl = 0, c = 0;
+  for (auto& kv : by_variant) nl += enqueue_tick(precision, kv.second, d_params, cs[c++ % n]);
+  for (int k = 0; k < n; ++k) {
+    (void)hipEventRecord(join[k], cs[k]);
+    (void)hipStreamWaitEvent(st, join[k], 0);
+  }
+  return nl;
+}
+
 int enqueue_fill(int precision, const std::vector<std::pair<int, const SweepParams*>>& ps,
                  const SweepParams* d_params, hipStream_t st) {
   int nl = 0;
@@ -859,6 +879,9 @@ struct MswServer {
   // (0: the plan default), z-stream steps per workgroup, schedule heuristic
   std::map<std::pair<int, int>, int> class_nac;
   int row_groups = 0, stream_target = 128, schedule_kind = 0;
+  // a tick's kernel variants run on their own streams, forked from st
+  hipStream_t chain_st[kChainStreams] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[kChainStreams] = {};
 };
 
 namespace {
@@ -1027,7 +1050,8 @@ void server_loop(MswServer* s) {
         if (a.next == a.s->nlaunch()) finishing.push_back((int)k);
       }
       if (!work.empty()) {
-        s->nlaunch += enqueue_tick(s->opt.precision, work, s->d_params, s->st);
+        s->nlaunch += enqueue_tick_forked(s->opt.precision, work, s->d_params, s->st,
+                                          s->chain_st, s->fork_ev, s->join_ev, kChainStreams);
         s->nticks++;
       }
       // norms of the problems whose last launch was in this tick, after it
@@ -1110,6 +1134,14 @@ int msw_server_create(const MswServerOptions* opt, MswServer** out) {
   if (e != hipSuccess) return fail(e, "hipGetDevice");
   if ((e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking)) != hipSuccess)
     return fail(e, "server stream");
+  for (int c = 0; c < kChainStreams; ++c) {
+    if ((e = hipStreamCreateWithFlags(&s->chain_st[c], hipStreamNonBlocking)) != hipSuccess)
+      return fail(e, "server stream");
+    if ((e = hipEventCreateWithFlags(&s->join_ev[c], hipEventDisableTiming)) != hipSuccess)
+      return fail(e, "server event");
+  }
+  if ((e = hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming)) != hipSuccess)
+    return fail(e, "server event");
   if ((e = hipMalloc(&s->arena_base, o.hbm_budget_bytes)) != hipSuccess)
     return fail(e, "server arena");
   s->arena.reset(o.hbm_budget_bytes);
@@ -1236,6 +1268,11 @@ int msw_server_destroy(MswServer* s) {
   if (s->d_res) (void)hipFree(s->d_res);
   if (s->h_res) (void)hipHostFree(s->h_res);
   if (s->st) (void)hipStreamDestroy(s->st);
+  for (int c = 0; c < kChainStreams; ++c) {
+    if (s->chain_st[c]) (void)hipStreamDestroy(s->chain_st[c]);
+    if (s->join_ev[c]) (void)hipEventDestroy(s->join_ev[c]);
+  }
+  if (s->fork_ev) (void)hipEventDestroy(s->fork_ev);
   delete s;
   return MSW_OK;
 }
