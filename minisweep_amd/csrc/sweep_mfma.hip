@@ -207,11 +207,26 @@ __host__ __device__ __forceinline__ unsigned coef_lut(int shift) {
 }
 
 // Waves per SIMD the register allocation aims for: two where the working set
-// fits in 256 registers (FP64 with 16 angles per wave, FP32 with up to 32),
-// so that one wave's matrix work overlaps the other's solve and loads.
-template <class PR, int NAC>
+// fits in 256 registers without spilling to scratch, so that one wave's matrix
+// work overlaps the other's solve and loads; else one (the compiler then
+// keeps the overflow in accumulation registers).  The shapes that fit were
+// read off the compiler's resource report for every instantiation: 16 angles
+// per wave in FP32; 32 in FP32 with one row group (or two with two angle
+// waves) below 16 moments, and in BF16 likewise at any moment count; 16 in
+// FP64 with several angle waves, one row group and below 16 moments.
+template <class PR, int NM, int NAC, int NWA, int NY>
 struct Occupancy {
-  static constexpr int value = sizeof(typename PR::T) * NAC <= 128 ? 2 : 1;
+  static constexpr int bytes = (int)sizeof(typename PR::T) * NAC;
+  static constexpr bool rows_ok = NY == 1 || (NWA == 2 && NY == 2 && NM < 16);
+  static constexpr int value =
+      bytes <= 64                                                          ? 2
+      : bytes <= 128 && sizeof(typename PR::T) == 4 && rows_ok &&
+              (NM < 16 || PR::kPacked)                                     ? 2
+      : bytes <= 128 && NWA >= 2 && NY == 1 && NM < 16                     ? 2
+                                                                           : 1;
+  // the many-problem kernel needs a few registers more (problem lookup): two
+  // waves only at 16 angles per wave in FP32 / BF16
+  static constexpr int multi = bytes <= 64 ? 2 : 1;
 };
 
 // One workgroup's work: tile item `it` of a sweep, energy groups
@@ -666,7 +681,8 @@ __device__ __forceinline__ void sweep_block(const SweepParams& p, int item_base,
 // Single sweep.
 template <class PR, int NM, int NAC, int NWA, int NY>
 __global__ __launch_bounds__(64 * NWA * NY) __attribute__((amdgpu_waves_per_eu(
-    Occupancy<PR, NAC>::value))) void k_sweep_mfma(SweepParams p, int item_base, int epw) {
+    Occupancy<PR, NM, NAC, NWA, NY>::value))) void k_sweep_mfma(SweepParams p, int item_base,
+                                                                int epw) {
   sweep_block<PR, NM, NAC, NWA, NY>(p, item_base, (int)blockIdx.x, epw);
 }
 
@@ -675,8 +691,8 @@ __global__ __launch_bounds__(64 * NWA * NY) __attribute__((amdgpu_waves_per_eu(
 // [start[k], start[k+1]) belong to it.
 template <class PR, int NM, int NAC, int NWA, int NY>
 __global__ __launch_bounds__(64 * NWA * NY) __attribute__((amdgpu_waves_per_eu(
-    Occupancy<PR, NAC>::value))) void k_sweep_mfma_multi(const SweepParams* __restrict__ probs,
-                                                       MultiLaunch ml) {
+    Occupancy<PR, NM, NAC, NWA, NY>::multi))) void k_sweep_mfma_multi(
+    const SweepParams* __restrict__ probs, MultiLaunch ml) {
   const int blk = (int)blockIdx.x;
   int k = 0;
   for (int step = kMaxSegments / 2; step > 0; step >>= 1)
