@@ -16,9 +16,12 @@ Prints ONE JSON line.  `value` is the FP64 sweep's GFLOP/s.  Also in it:
   roofline     the dominant kernel (k_sweep_mfma) against the FP64 peak, from
                per-launch HIP events recorded on the sweep's stream during the
                timed steps; HBM traffic from the committed rocprofv3 summary.
-  cpu_baseline the CPU oracle (oracle/, a restatement of the reference
-               algorithm) timed on this host over a bounded sample of the same
-               shape: a few energy groups, one sweep, OpenMP over groups.
+  cpu_baseline the reference's own CPU sweep (its serial KBA build, compiled
+               from its sources into oracle/_ref/ by oracle/Makefile) timed on
+               this host's core over one energy group of the same shape, one
+               sweep; beside it ("port") the CPU oracle (oracle/, the
+               restatement of the reference algorithm) with OpenMP over a few
+               energy groups.  Without oracle/_ref the oracle alone.
 """
 from __future__ import annotations
 
@@ -82,6 +85,34 @@ def cpu_baseline(ncx, ncy, ncz, nm, na, nu, threads):
         "sample": (f"{ncx}x{ncy}x{ncz} cells, ne={ne_s} groups (of the workload's "
                    f"energy groups), na={na}, nm={nm}, nu={nu}, 1 sweep, "
                    f"{t:.1f} s, exact={ok}"),
+    }
+
+
+def cpu_baseline_reference(ncx, ncy, ncz, nm, na, nu):
+    """The reference's serial sweep driver (oracle/_ref/nm<NM>/sweep) on one
+    energy group of the shape: its own result line gives time and GF/s."""
+    import re
+    import subprocess
+    exe = os.path.join(REPO, "oracle", "_ref", f"nm{nm}", "sweep")
+    if nu != 4 or not os.access(exe, os.X_OK):  # the reference fixes NU = 4
+        return None
+    cmd = [exe, "--ncell_x", str(ncx), "--ncell_y", str(ncy), "--ncell_z", str(ncz),
+           "--ne", "1", "--na", str(na), "--niterations", "1"]
+    try:
+        out = subprocess.run(cmd, capture_output=True, text=True, timeout=300).stdout
+    except (OSError, subprocess.TimeoutExpired):
+        return None
+    m = re.search(r"Normsq result: (\S+)\s+diff: (\S+)\s+(\w+)\s+time: (\S+)\s+GF/s: (\S+)", out)
+    if not m:
+        return None
+    return {
+        "value": float(m.group(5)),
+        "unit": "GFLOP/s",
+        "cores": 1,
+        "kind": "reference",
+        "sample": (f"reference serial KBA sweep, {ncx}x{ncy}x{ncz} cells, ne=1 group (of the "
+                   f"workload's energy groups), na={na}, nm={nm}, nu=4, 1 sweep, "
+                   f"{float(m.group(4)):.1f} s, {m.group(3)}"),
     }
 
 
@@ -160,6 +191,12 @@ def main():
               "supported (the sweep would shard by energy group across GPUs, "
               "see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    # the reference's CPU run goes first, before this process touches the GPU
+    ref_cpu = None
+    if not args.no_cpu_baseline:
+        ref_cpu = cpu_baseline_reference(args.ncell, args.ncell, args.ncell, args.nm, args.na,
+                                         args.nu)
 
     import torch
     import minisweep_amd as msw
@@ -248,6 +285,9 @@ def main():
         threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "16"))
         threads = max(1, min(threads, 16))
         cpu = cpu_baseline(n, n, n, args.nm, args.na, args.nu, threads)
+        if ref_cpu is not None:
+            ref_cpu["port"] = cpu
+            cpu = ref_cpu
 
     out = {
         "metric": METRIC,
