@@ -33,8 +33,10 @@ enum {
 /* Arithmetic of a sweep. */
 enum {
   MSW_PREC_FP64 = 0,        /* plain FP64, parity path */
-  MSW_PREC_MIXED_FP32 = 1,  /* FP32 contractions + FP32 face recurrence */
-  MSW_PREC_MIXED_BF16 = 2   /* BF16 contractions (FP32 accumulate) + FP32 faces */
+  MSW_PREC_MIXED_FP32 = 1,  /* FP32 contractions + FP32 face recurrence and octant
+                               sum; FP64 state, last-visit sum and norms */
+  MSW_PREC_MIXED_BF16 = 2   /* as MIXED_FP32 with BF16 contractions (FP32
+                               accumulate): exact only on BF16-exact inputs */
 };
 
 /* Problem geometry of one sweep (reference Dimensions + runtime nu). */
@@ -68,7 +70,8 @@ enum {
 typedef struct {
   int precision;       /* MSW_PREC_* */
   int engine;          /* MSW_ENGINE_* */
-  int angles_per_wave; /* 0: chosen from na; MFMA: 16, 32 or 64; VALU: 8, 16 or 32 */
+  int angles_per_wave; /* 0: chosen from na; MFMA: 16, 32 or 64 (64 in FP64
+                          spills registers: correct, slow); VALU: 8, 16 or 32 */
   int groups_per_workgroup; /* MFMA: energy groups streamed through one tile
                                pipeline (0: chosen per launch from ncell_z) */
   int tile_rows;            /* MFMA: cell rows per tile, 4, 8 or 16 (0: chosen) */

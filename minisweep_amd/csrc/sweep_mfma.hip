@@ -3,11 +3,13 @@
 // accumulation, FP32 faces and solve, FP64 input state; vo summed over the
 // octants in FP32 and written in FP64 by each tile's last octant visit).
 //
-// One wave owns a tile of 16 (x) by 4 (y) cells for one (octant, energy
-// group, unknown, chunk of NAC angles) and walks the whole z extent.  Lane
-// l = j + 16g owns cell (x = j, y = b) of every cell block b = 0..3; at step
-// t that cell is at octant-frame z = t - j - b, so the 64 cells of a step lie
-// on one diagonal hyperplane of the tile.
+// One wave owns 16 (x) by 4 (y) cells for one (octant, run of energy groups,
+// unknown, chunk of NAC angles) and walks the z extent of every group of the
+// run as one stream.  Lane l = j + 16g owns cell (x = j, y = b) of every cell
+// block b = 0..3; at step t that cell is at stream position t - j - b, so the
+// 64 cells of a step lie on one diagonal hyperplane.  A workgroup is NWA
+// angle-chunk waves times NY row groups (a 16 x 4NY tile); row group w runs
+// 4w steps behind row group w-1 and takes its y inflow from it through LDS.
 //
 //   * m2a: VS[angle][cell] = AT[angle][moment] . VI[moment][cell] as 16x16
 //     MFMAs.  AT is the A operand (per octant, in registers for the whole
@@ -17,7 +19,8 @@
 //     (j, g), register r.  The solve runs in that layout: the z face is the
 //     lane's own value from the previous step, the y face block b-1's value of
 //     the same lane, the x face lane j-1's (DPP row shift).  Lane j = 0 gets
-//     the tile's x inflow and block 0 the y inflow, read from the face planes
+//     the tile's x inflow (loaded one value per lane of the row, gathered by
+//     a DPP row rotation) and block 0 the y inflow, read from the face planes
 //     one step ahead.
 //   * The carried face value is the solved angular value itself; the
 //     reference's faces hold it times (1 + octant), which cancels against its
