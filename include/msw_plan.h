@@ -75,6 +75,10 @@ typedef struct {
   int groups_per_workgroup; /* MFMA: energy groups streamed through one tile
                                pipeline (0: chosen per launch from ncell_z) */
   int tile_rows;            /* MFMA: cell rows per tile, 4, 8 or 16 (0: chosen) */
+  int z_block;              /* MFMA, FP64: sweep the grid z_block planes at a time
+                               (must divide ncell_z; 0: all planes at once).  The
+                               reference's --nblock_z; needed by msw_plan_sweep_host
+                               for states larger than HBM */
 } MswPlanOptions;
 
 void msw_plan_default_options(MswPlanOptions* opt);
@@ -99,6 +103,18 @@ int msw_plan_num_launches(const MswPlan* plan);
 int msw_plan_sweep(MswPlan* plan, const double* vi, double* vo,
                    const double* a_from_m, const double* m_from_a,
                    void* stream);
+
+/* Out-of-core sweep (this library; the reference's z-block streaming,
+ * src/3_sweeper/sweeper_kba_c.h:719-781): vi_host and vo_host are HOST arrays
+ * in the state layout, pinned (hipHostMalloc) for the copies to overlap the
+ * sweep.  The state goes through HBM one z block at a time, each block's vi
+ * (and, in the second pass, its partial vo) copied in while the previous
+ * block is swept, its vo copied back after: two passes of four octants, 6
+ * state volumes over PCIe per sweep.  Device memory: two slabs of vi and vo
+ * of one block each, plus the plan.  a_from_m, m_from_a are device arrays.
+ * Enqueued on stream; synchronize it before reading vo_host. */
+int msw_plan_sweep_host(MswPlan* plan, const double* vi_host, double* vo_host,
+                        const double* a_from_m, const double* m_from_a, void* stream);
 
 /* Per-launch timing (this library): when enabled, msw_plan_sweep brackets
  * each sweep-kernel launch with hipEvents on the sweep's stream. */

@@ -39,7 +39,7 @@ class MswDims(ctypes.Structure):
 class MswPlanOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("engine", ctypes.c_int),
                 ("angles_per_wave", ctypes.c_int), ("groups_per_workgroup", ctypes.c_int),
-                ("tile_rows", ctypes.c_int)]
+                ("tile_rows", ctypes.c_int), ("z_block", ctypes.c_int)]
 
 
 class MswBatchOptions(ctypes.Structure):
@@ -89,6 +89,7 @@ def lib():
         L.msw_plan_workspace_bytes.restype = ctypes.c_size_t
         L.msw_plan_num_launches.argtypes = [vp]
         L.msw_plan_sweep.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.msw_plan_sweep_host.argtypes = [vp, vp, vp, vp, vp, vp]
         L.msw_plan_destroy.argtypes = [vp]
         L.msw_plan_set_timing.argtypes = [vp, ctypes.c_int]
         L.msw_plan_last_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_float),
@@ -174,7 +175,7 @@ class Plan:
 
     def __init__(self, d: MswDims, precision: int = PREC_FP64,
                  engine: Optional[int] = None, angles_per_wave: int = 0,
-                 groups_per_workgroup: int = 0, tile_rows: int = 0):
+                 groups_per_workgroup: int = 0, tile_rows: int = 0, z_block: int = 0):
         self.d = d
         self.precision = precision
         o = MswPlanOptions()
@@ -185,6 +186,7 @@ class Plan:
         o.angles_per_wave = angles_per_wave
         o.groups_per_workgroup = groups_per_workgroup
         o.tile_rows = tile_rows
+        o.z_block = z_block
         self.engine = o.engine
         h = ctypes.c_void_p()
         _check(lib().msw_plan_create_ex(ctypes.byref(d), ctypes.byref(o), ctypes.byref(h)),
@@ -204,6 +206,15 @@ class Plan:
         _check(lib().msw_plan_sweep(self._h, _dptr(vi), _dptr(vo), _dptr(a_from_m),
                                     _dptr(m_from_a), _stream_ptr(stream)),
                "msw_plan_sweep")
+
+    def sweep_host(self, vi_host, vo_host, a_from_m, m_from_a, stream=None):
+        """vo_host <- S(vi_host) with the state in host memory (CPU tensors,
+        pinned for overlap), streamed through HBM by z block; the matrices are
+        device tensors.  Enqueued on stream."""
+        _check(lib().msw_plan_sweep_host(self._h, ctypes.c_void_p(vi_host.data_ptr()),
+                                         ctypes.c_void_p(vo_host.data_ptr()), _dptr(a_from_m),
+                                         _dptr(m_from_a), _stream_ptr(stream)),
+               "msw_plan_sweep_host")
 
     def set_timing(self, enable: bool = True):
         _check(lib().msw_plan_set_timing(self._h, int(enable)), "msw_plan_set_timing")

@@ -78,6 +78,16 @@ struct SweepParams {
   int nx, ny, nz, ne, na, nu;
   int na_pad;  // na rounded up to whole angle chunks (matrix/face stride)
   int ntx, nty;
+  // z blocks (MFMA engine, FP64): the sweep runs one block of nz planes at a
+  // time, vi/vo pointing at the block.  z_off is the block's first plane in
+  // the grid and nz_grid the grid's extent (0: not blocked, nz is the grid).
+  // fz holds the z faces between blocks, [4][nu][ne][ny][nx][na_pad] indexed
+  // by octant & 3, angles in lane order; z_in: this block takes its z inflow
+  // from fz (it is not the first block in the octant's direction), z_out: it
+  // writes its last plane's outflow there (not the last block).
+  int z_off = 0, nz_grid = 0;
+  void* fz = nullptr;
+  int z_in = 0, z_out = 0;
 };
 
 // A launch that runs segments of several sweeps: segment k covers

@@ -66,7 +66,12 @@ int default_mfma_ny(int precision, int nwa, int ncell_y);
 
 // Host list schedule of (octant, tile) visits for ntx x nty tiles (kind 0:
 // most-remaining octant first; 1: the reference's octant order).
-std::vector<std::vector<TileItem>> build_schedule(int ntx, int nty, int kind);
+// Launches of (octant, tile) visits: every octant in oct_mask visits every
+// tile once, upwind tiles first; one visit per physical tile per launch.
+// Flags mark a tile's first visit (touched count 0) and last (count 7), the
+// counts starting at touched0 (visits made by earlier passes).
+std::vector<std::vector<TileItem>> build_schedule(int ntx, int nty, int kind,
+                                                  unsigned oct_mask = 0xFF, int touched0 = 0);
 
 // Small kernels of sweep_plan.hip.
 void launch_pack(const double* afm, const double* mfa, double* at, double* mf, int nm, int na,

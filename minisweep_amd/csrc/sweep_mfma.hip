@@ -300,7 +300,10 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
     sc[b] = cxo + (size_t)yb[b] * nx * nu * NM;
     cst[b] = x + 3 * yb[b] + 2;
   }
-  const int zb = dz > 0 ? -1 : nz;
+  // z blocks: this block's planes are grid planes z_off .. z_off + nz - 1;
+  // the domain's z boundary is outside the grid, not the block
+  const int zb = dz > 0 ? -1 : (p.nz_grid ? p.nz_grid : nz);
+  const int zoff = p.z_off;
 
   // ---- LDS: vo partial sums [2][NY][kTY][NWA][64][4], y faces between row
   // groups [2][NY-1][NWA][NAB][64][4] --
@@ -319,6 +322,12 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
            4 * g;
   T* fxz = reinterpret_cast<T*>(p.fxz) + (face_ou * nx * nze + (size_t)ie0 * nz) * nap + ia0 +
            4 * g + x * fplane;  // this lane's column
+  // z faces between z blocks: [octant & 3][iu][group][y][x][angle], the lane's
+  // column; fz_cell(b) addresses block b's cell in its current group
+  T* fzl = p.fz ? reinterpret_cast<T*>(p.fz) +
+                      ((size_t)((oct & 3) * nu + iu) * p.ne + ie0) * ny * nx * nap +
+                      (size_t)x * nap + ia0 + 4 * g
+                : nullptr;
 
   // ---- per-octant operands, in registers for the whole launch --------------
   const int moff = MSTEP * g;
@@ -386,6 +395,10 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
     zs[b] = -j - b - kTY * wyi;
     es[b] = 0;
   }
+  auto fz_cell = [&](int b) {
+    const int ec = es[b] < nge ? es[b] : nge - 1;
+    return fzl + ((size_t)ec * ny + yb[b]) * nx * nap;
+  };
   int zn[kTY], en[kTY];  // next step's stream position per block
   size_t sn[kTY];        // next step's state offsets
   auto advance = [&]() {
@@ -481,7 +494,7 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
       ent[b] = zs[b] == 0 && es[b] < nge;
       const int zc = zs[b] < 0 ? 0 : zs[b];
       zgs[b] = dz > 0 ? zc : nz - 1 - zc;
-      sh[b] = ((cst[b] + 7 * zgs[b]) & 3) * 4;
+      sh[b] = ((cst[b] + 7 * (zoff + zgs[b])) & 3) * 4;
     }
 
     acc4 vacc[kTY];
@@ -534,11 +547,18 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
 #pragma unroll
         for (int r = 0; r < 4; ++r) rz[r] = F[a][b][r];
         if (any_ent[b]) {  // some lane of this block starts a group
-          const T sc = ent[b] ? (T)ldexp((double)(sf_energy(ie0 + es[b]) * su),
-                                         sf_space_exp(x, yb[b], zb))
-                              : (T)0;
+          if (p.z_in) {  // inflow: the previous z block's last plane
+            T zi[4];
+            load4(fz_cell(b) + 16 * a, zi);
 #pragma unroll
-          for (int r = 0; r < 4; ++r) rz[r] = ent[b] ? (T)iaf[a][r] * sc : rz[r];
+            for (int r = 0; r < 4; ++r) rz[r] = ent[b] ? zi[r] : rz[r];
+          } else {
+            const T sc = ent[b] ? (T)ldexp((double)(sf_energy(ie0 + es[b]) * su),
+                                           sf_space_exp(x, yb[b], zb))
+                                : (T)0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) rz[r] = ent[b] ? (T)iaf[a][r] * sc : rz[r];
+          }
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -595,6 +615,15 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
       for (int b = 0; b < kTY; ++b)
         if (kTY * wyi + b == wy - 1 && val[b]) {
           T* dst = fxz + (fo0 - b * nap);
+#pragma unroll
+          for (int a = 0; a < NAB; ++a) store4(dst + 16 * a, F[a][b]);
+        }
+    }
+    if (p.z_out) {  // the block's last plane is the next block's z inflow
+#pragma unroll
+      for (int b = 0; b < kTY; ++b)
+        if (val[b] && zs[b] == nz - 1) {
+          T* dst = fz_cell(b);
 #pragma unroll
           for (int a = 0; a < NAB; ++a) store4(dst + 16 * a, F[a][b]);
         }
@@ -733,7 +762,7 @@ __device__ __forceinline__ void fill_row(const SweepParams& p, int row) {
   for (int i = threadIdx.x; i < (int)len; i += blockDim.x) {
     const int pos = i % nap, q = i / nap;
     const int zq = q % nz, ie = q / nz;
-    const int z = dz > 0 ? zq : nz - 1 - zq;
+    const int z = p.z_off + (dz > 0 ? zq : nz - 1 - zq);  // grid plane
     const int ia = 16 * (pos >> 4) + PR::row((pos & 15) >> 2, pos & 3);
     const int e = xface ? sf_space_exp((oct & 1) ? p.nx : -1, c, z)
                         : sf_space_exp(c, (oct & 2) ? p.ny : -1, z);

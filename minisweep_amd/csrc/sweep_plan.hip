@@ -465,16 +465,26 @@ __global__ __launch_bounds__(kNormThreads) void k_norms_final(
 // the reference's fixed octant order {0,4,2,6,3,7,1,5}
 // (src/3_sweeper/stepscheduler_kba.c:155).  Returns the launches in order;
 // marks the first and the last visit of every physical tile.
-std::vector<std::vector<TileItem>> build_schedule(int ntx, int nty, int kind) {
+std::vector<std::vector<TileItem>> build_schedule(int ntx, int nty, int kind, unsigned oct_mask,
+                                                  int touched0) {
   static const int kRefOrder[kNoct] = {0, 4, 2, 6, 3, 7, 1, 5};
   std::vector<std::vector<TileItem>> launches;
   std::vector<std::vector<int>> done(kNoct, std::vector<int>(ntx * nty, 0));
-  std::vector<int> remaining(kNoct, ntx * nty);
-  std::vector<int> touched(ntx * nty, 0);  // visits so far per physical tile
-  int left = kNoct * ntx * nty;
+  std::vector<int> remaining(kNoct, 0);
+  int left = 0;
+  for (int o = 0; o < kNoct; ++o)
+    if (oct_mask >> o & 1) {
+      remaining[o] = ntx * nty;
+      left += ntx * nty;
+    }
+  // visits so far per physical tile (touched0 from earlier passes)
+  std::vector<int> touched(ntx * nty, touched0);
   while (left > 0) {
-    std::vector<int> order(kNoct);
-    for (int o = 0; o < kNoct; ++o) order[o] = kind == 1 ? kRefOrder[o] : o;
+    std::vector<int> order;
+    for (int o = 0; o < kNoct; ++o) {
+      const int oc = kind == 1 ? kRefOrder[o] : o;
+      if (oct_mask >> oc & 1) order.push_back(oc);
+    }
     if (kind != 1)
       std::stable_sort(order.begin(), order.end(),
                        [&](int a, int b) { return remaining[a] > remaining[b]; });
@@ -571,16 +581,28 @@ struct MswPlan {
   // optional per-launch timing (hipEvents on the sweep's stream)
   int timing;
   std::vector<hipEvent_t> ev_start, ev_stop;
+  // z blocks: zb planes per block, nblk blocks; with nblk > 1 the launches
+  // are two passes, octants 0-3 (upward in z) then 4-7, pass_split launches
+  // in the first; fz the z faces between blocks
+  int zb = 0, nblk = 1, pass_split = 0;
+  void* fz = nullptr;
+  // host-streamed sweeps: device slabs (double-buffered), copy stream, events
+  double* slab_vi[2] = {nullptr, nullptr};
+  double* slab_vo[2] = {nullptr, nullptr};
+  int nslot = 0;
+  hipStream_t cst = nullptr;
+  hipEvent_t ev_in[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr};
 };
 
 template <int NM, int NAC>
-static int launch_all(MswPlan* p, const SweepParams& sp, hipStream_t st) {
+static int launch_all(MswPlan* p, const SweepParams& sp, hipStream_t st, size_t l0, size_t l1) {
   const dim3 block(64 * p->nwa);
-  for (size_t l = 0; l < p->launch_off.size(); ++l) {
+  for (size_t l = l0; l < l1; ++l) {
     const int epw = p->engine == MSW_ENGINE_MFMA ? p->launch_epw[l] : 1;
     const int nec = (p->d.ne + epw - 1) / epw;
     const dim3 grid((unsigned)p->launch_cnt[l] * nec * p->d.nu);
-    if (p->timing) HIP_TRY(hipEventRecord(p->ev_start[l], st));
+    const bool timed = p->timing && l0 == 0 && l1 == p->launch_off.size();
+    if (timed) HIP_TRY(hipEventRecord(p->ev_start[l], st));
     if (p->engine == MSW_ENGINE_MFMA) {
       LaunchSpec ls;
       ls.item_base = p->launch_off[l];
@@ -596,18 +618,27 @@ static int launch_all(MswPlan* p, const SweepParams& sp, hipStream_t st) {
       hipLaunchKernelGGL((k_sweep_fp64<NM, NAC>), grid, block, p->lds_bytes, st, sp,
                          p->launch_off[l]);
     }
-    if (p->timing) HIP_TRY(hipEventRecord(p->ev_stop[l], st));
+    if (timed) HIP_TRY(hipEventRecord(p->ev_stop[l], st));
   }
   HIP_TRY(hipGetLastError());
   return MSW_OK;
 }
 
 template <int NM>
-static int launch_nm(MswPlan* p, const SweepParams& sp, hipStream_t st) {
+static int launch_nm(MswPlan* p, const SweepParams& sp, hipStream_t st, size_t l0, size_t l1) {
   switch (p->nac) {
-    case 8: return launch_all<NM, 8>(p, sp, st);
-    case 16: return launch_all<NM, 16>(p, sp, st);
-    default: return launch_all<NM, 32>(p, sp, st);
+    case 8: return launch_all<NM, 8>(p, sp, st, l0, l1);
+    case 16: return launch_all<NM, 16>(p, sp, st, l0, l1);
+    default: return launch_all<NM, 32>(p, sp, st, l0, l1);
+  }
+}
+
+// Launches [l0, l1) of a plan's schedule with parameters sp.
+static int launch_range(MswPlan* p, const SweepParams& sp, hipStream_t st, size_t l0, size_t l1) {
+  switch (p->d.nm) {
+    case 4: return launch_nm<4>(p, sp, st, l0, l1);
+    case 8: return launch_nm<8>(p, sp, st, l0, l1);
+    default: return launch_nm<16>(p, sp, st, l0, l1);
   }
 }
 
@@ -650,6 +681,7 @@ void msw_plan_default_options(MswPlanOptions* o) {
   o->angles_per_wave = 0;
   o->groups_per_workgroup = 0;
   o->tile_rows = 0;
+  o->z_block = 0;
 }
 
 int msw_plan_create(const MswDims* d, int precision, MswPlan** out) {
@@ -705,13 +737,31 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
                    "", d->na);
   }
   p->na_pad = p->nwa * p->nac;
-  if (p->engine == MSW_ENGINE_MFMA && !mfma_face_offsets_fit(*d, p->na_pad)) {
-    delete p;
-    return set_error(MSW_ERR_UNSUPPORTED, "face planes too large for 32-bit offsets");
-  }
   if (opt->groups_per_workgroup < 0) {
     delete p;
     return set_error(MSW_ERR_INVALID, "groups_per_workgroup must be >= 0");
+  }
+  p->zb = d->ncell_z;
+  if (opt->z_block > 0 && opt->z_block < d->ncell_z) {
+    if (p->engine != MSW_ENGINE_MFMA || precision != MSW_PREC_FP64) {
+      delete p;
+      return set_error(MSW_ERR_UNSUPPORTED, "z blocks need the MFMA engine in FP64");
+    }
+    if (d->ncell_z % opt->z_block != 0) {
+      delete p;
+      return set_error(MSW_ERR_INVALID, "z_block must divide ncell_z");
+    }
+    p->zb = opt->z_block;
+    p->nblk = d->ncell_z / opt->z_block;
+  } else if (opt->z_block < 0) {
+    delete p;
+    return set_error(MSW_ERR_INVALID, "z_block must be >= 0");
+  }
+  MswDims dblk = *d;  // the geometry one kernel launch sees
+  dblk.ncell_z = p->zb;
+  if (p->engine == MSW_ENGINE_MFMA && !mfma_face_offsets_fit(dblk, p->na_pad)) {
+    delete p;
+    return set_error(MSW_ERR_UNSUPPORTED, "face planes too large for 32-bit offsets");
   }
   // tile of one wave: 8x8 cells (VALU engine), 16x4 (MFMA engine)
   const int tx = p->engine == MSW_ENGINE_MFMA ? mfma_tile_x() : kTile;
@@ -736,13 +786,19 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
     p->lds_bytes = sizeof(double) * ((size_t)p->nwa * 2 * kTile * (p->nac + 1) +
                                      (p->nwa > 1 ? (size_t)p->nwa * d->nm * 64 : 0));
   auto launches = build_schedule(p->ntx, p->nty, 0);
+  if (p->nblk > 1) {  // two passes of four octants, each over every z block
+    launches = build_schedule(p->ntx, p->nty, 0, 0x0F, 0);
+    p->pass_split = (int)launches.size();
+    auto up = build_schedule(p->ntx, p->nty, 0, 0xF0, 4);
+    launches.insert(launches.end(), up.begin(), up.end());
+  }
   // energy groups per workgroup, per launch: streamed while the launch
   // keeps its target wave count, or as the caller fixed it
   for (auto& l : launches) {
     const long long w1 = (long long)l.size() * d->nu * p->nwa * p->ny;
     p->launch_epw.push_back(opt->groups_per_workgroup > 0
                                 ? std::min(opt->groups_per_workgroup, d->ne)
-                                : launch_mfma_epw(default_mfma_epw(d->ncell_z, d->ne), d->ne, w1,
+                                : launch_mfma_epw(default_mfma_epw(p->zb, d->ne), d->ne, w1,
                                                   precision));
   }
   std::vector<TileItem> flat;
@@ -754,8 +810,10 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
   // matrices in whole angle chunks (zero rows past na), plus one chunk of
   // slack so that no scalar load can leave the buffer
   const size_t nat = ((size_t)kNoct * p->na_pad + p->nac) * d->nm;
-  const size_t nfyz = (size_t)kNoct * d->ne * d->nu * d->ncell_y * d->ncell_z * p->na_pad;
-  const size_t nfxz = (size_t)kNoct * d->ne * d->nu * d->ncell_x * d->ncell_z * p->na_pad;
+  const size_t nfyz = (size_t)kNoct * d->ne * d->nu * d->ncell_y * p->zb * p->na_pad;
+  const size_t nfxz = (size_t)kNoct * d->ne * d->nu * d->ncell_x * p->zb * p->na_pad;
+  const size_t nfz = p->nblk > 1 ? (size_t)4 * d->nu * d->ne * d->ncell_y * d->ncell_x * p->na_pad
+                                 : 0;
   const size_t fsz =
       p->engine == MSW_ENGINE_MFMA ? mfma_face_elem_bytes(precision) : sizeof(double);
   p->bytes = sizeof(double) * (2 * nat + 2 * kNormBlocks) + fsz * (nfyz + nfxz) +
@@ -767,6 +825,10 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
   if (e == hipSuccess) e = hipMemset(p->mf, 0, sizeof(double) * nat);
   if (e == hipSuccess) e = hipMalloc(&p->fyz, fsz * nfyz);
   if (e == hipSuccess) e = hipMalloc(&p->fxz, fsz * nfxz);
+  if (e == hipSuccess && nfz) {
+    e = hipMalloc(&p->fz, sizeof(double) * nfz);
+    p->bytes += sizeof(double) * nfz;
+  }
   if (e == hipSuccess && p->engine == MSW_ENGINE_MFMA && precision != MSW_PREC_FP64) {
     e = hipMalloc(&p->vo32, sizeof(float) * msw_size_state(d));
     p->bytes += sizeof(float) * msw_size_state(d);
@@ -842,7 +904,67 @@ int msw_plan_destroy(MswPlan* p) {
   if (p->vi32) (void)hipFree(p->vi32);
   if (p->norm_part) (void)hipFree(p->norm_part);
   if (p->items) (void)hipFree(p->items);
+  if (p->fz) (void)hipFree(p->fz);
+  if (p->cst) (void)hipStreamSynchronize(p->cst);
+  for (int k = 0; k < 2; ++k) {
+    if (p->slab_vi[k]) (void)hipFree(p->slab_vi[k]);
+    if (p->slab_vo[k]) (void)hipFree(p->slab_vo[k]);
+    if (p->ev_in[k]) (void)hipEventDestroy(p->ev_in[k]);
+    if (p->ev_done[k]) (void)hipEventDestroy(p->ev_done[k]);
+  }
+  if (p->cst) (void)hipStreamDestroy(p->cst);
   delete p;
+  return MSW_OK;
+}
+
+// The launches of one sweep on device-resident vi, vo.  A plan with z blocks
+// runs two passes (octants 0-3 upward in z, then 4-7 downward), each over the
+// blocks in its direction: a block's boundary fill, then the pass's launches
+// on that block's planes.
+static int sweep_device(MswPlan* p, const double* vi, double* vo, hipStream_t st) {
+  const MswDims& d = p->d;
+  SweepParams sp;
+  sp.vi = vi;
+  sp.vo = vo;
+  sp.at = p->at;
+  sp.mf = p->mf;
+  sp.fyz = p->fyz;
+  sp.fxz = p->fxz;
+  sp.vo32 = p->vo32;
+  sp.vi32 = p->vi32;
+  sp.items = p->items;
+  sp.nx = d.ncell_x;
+  sp.ny = d.ncell_y;
+  sp.nz = d.ncell_z;
+  sp.ne = d.ne;
+  sp.na = d.na;
+  sp.nu = d.nu;
+  sp.na_pad = p->na_pad;
+  sp.ntx = p->ntx;
+  sp.nty = p->nty;
+  if (p->nblk == 1) {
+    if (p->vi32) launch_to_f32(vi, p->vi32, msw_size_state(&d), st);
+    if (p->engine == MSW_ENGINE_MFMA) launch_mfma_fill(p->precision, sp, st);
+    return launch_range(p, sp, st, 0, p->launch_off.size());
+  }
+  const size_t slab = msw_size_state(&d) / p->nblk;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int kk = 0; kk < p->nblk; ++kk) {
+      const int k = pass == 0 ? kk : p->nblk - 1 - kk;
+      SweepParams b = sp;
+      b.vi = vi + (size_t)k * slab;
+      b.vo = vo + (size_t)k * slab;
+      b.nz = p->zb;
+      b.z_off = k * p->zb;
+      b.nz_grid = d.ncell_z;
+      b.fz = p->fz;
+      b.z_in = kk > 0;
+      b.z_out = kk < p->nblk - 1;
+      launch_mfma_fill(p->precision, b, st);
+      const int rc = launch_range(p, b, st, pass == 0 ? 0 : p->pass_split,
+                                  pass == 0 ? p->pass_split : p->launch_off.size());
+      if (rc) return rc;
+    }
   return MSW_OK;
 }
 
@@ -856,32 +978,108 @@ int msw_plan_sweep(MswPlan* p, const double* vi, double* vo,
   const int n = kNoct * d.nm * d.na;
   hipLaunchKernelGGL(k_pack_matrices, dim3((n + 255) / 256), dim3(256), 0, st,
                      a_from_m, m_from_a, p->at, p->mf, d.nm, d.na, p->na_pad);
-  SweepParams sp;
-  sp.vi = vi;
-  sp.vo = vo;
-  sp.at = p->at;
-  sp.mf = p->mf;
-  sp.fyz = p->fyz;
-  sp.fxz = p->fxz;
-  sp.vo32 = p->vo32;
-  sp.vi32 = p->vi32;
-  if (p->vi32) launch_to_f32(vi, p->vi32, msw_size_state(&d), st);
-  sp.items = p->items;
-  sp.nx = d.ncell_x;
-  sp.ny = d.ncell_y;
-  sp.nz = d.ncell_z;
-  sp.ne = d.ne;
-  sp.na = d.na;
-  sp.nu = d.nu;
-  sp.na_pad = p->na_pad;
-  sp.ntx = p->ntx;
-  sp.nty = p->nty;
-  if (p->engine == MSW_ENGINE_MFMA) launch_mfma_fill(p->precision, sp, st);
-  switch (d.nm) {
-    case 4: return launch_nm<4>(p, sp, st);
-    case 8: return launch_nm<8>(p, sp, st);
-    default: return launch_nm<16>(p, sp, st);
+  return sweep_device(p, vi, vo, st);
+}
+
+int msw_plan_sweep_host(MswPlan* p, const double* vi_host, double* vo_host,
+                        const double* a_from_m, const double* m_from_a, void* stream) {
+  if (!p || !vi_host || !vo_host || !a_from_m || !m_from_a)
+    return set_error(MSW_ERR_INVALID, "null argument to msw_plan_sweep_host");
+  if (vi_host == vo_host) return set_error(MSW_ERR_INVALID, "vi and vo must differ");
+  hipStream_t st = (hipStream_t)stream;
+  const MswDims& d = p->d;
+  const size_t slab = msw_size_state(&d) / p->nblk;
+  if (!p->nslot) {  // first call: slabs, copy stream, events
+    p->nslot = p->nblk > 1 ? 2 : 1;
+    for (int k = 0; k < p->nslot; ++k) {
+      HIP_TRY(hipMalloc(&p->slab_vi[k], sizeof(double) * slab));
+      HIP_TRY(hipMalloc(&p->slab_vo[k], sizeof(double) * slab));
+      HIP_TRY(hipEventCreateWithFlags(&p->ev_in[k], hipEventDisableTiming));
+      HIP_TRY(hipEventCreateWithFlags(&p->ev_done[k], hipEventDisableTiming));
+      p->bytes += 2 * sizeof(double) * slab;
+    }
+    HIP_TRY(hipStreamCreateWithFlags(&p->cst, hipStreamNonBlocking));
   }
+  const int n = kNoct * d.nm * d.na;
+  hipLaunchKernelGGL(k_pack_matrices, dim3((n + 255) / 256), dim3(256), 0, st,
+                     a_from_m, m_from_a, p->at, p->mf, d.nm, d.na, p->na_pad);
+  // steps: (pass, block); one step (pass -1) for an unblocked plan
+  struct Step { int pass, kk, k; };
+  std::vector<Step> steps;
+  if (p->nblk == 1)
+    steps.push_back(Step{-1, 0, 0});
+  else
+    for (int pass = 0; pass < 2; ++pass)
+      for (int kk = 0; kk < p->nblk; ++kk)
+        steps.push_back(Step{pass, kk, pass == 0 ? kk : p->nblk - 1 - kk});
+  const size_t bytes = sizeof(double) * slab;
+  // host -> device for step i into its slot: vi, and vo summed so far in the
+  // second pass
+  auto copy_in = [&](size_t i) -> int {
+    const Step& s = steps[i];
+    const int sl = (int)(i % p->nslot);
+    HIP_TRY(hipMemcpyAsync(p->slab_vi[sl], vi_host + (size_t)s.k * slab, bytes,
+                           hipMemcpyHostToDevice, p->cst));
+    if (s.pass == 1)
+      HIP_TRY(hipMemcpyAsync(p->slab_vo[sl], vo_host + (size_t)s.k * slab, bytes,
+                             hipMemcpyHostToDevice, p->cst));
+    HIP_TRY(hipEventRecord(p->ev_in[sl], p->cst));
+    return MSW_OK;
+  };
+  // Copies run on the plan's copy stream: the next step's inputs go in while
+  // this step computes, except where the next step is the same block (the
+  // turn between passes), whose vo must come back first.
+  int rc = copy_in(0);
+  if (rc) return rc;
+  for (size_t i = 0; i < steps.size(); ++i) {
+    const Step& s = steps[i];
+    const int sl = (int)(i % p->nslot);
+    const bool next = i + 1 < steps.size();
+    const bool turn = next && steps[i + 1].k == s.k;
+    if (next && !turn && (rc = copy_in(i + 1))) return rc;
+    HIP_TRY(hipStreamWaitEvent(st, p->ev_in[sl], 0));
+    if (s.pass < 0) {
+      rc = sweep_device(p, p->slab_vi[sl], p->slab_vo[sl], st);
+    } else {
+      SweepParams b;
+      b.vi = p->slab_vi[sl];
+      b.vo = p->slab_vo[sl];
+      b.at = p->at;
+      b.mf = p->mf;
+      b.fyz = p->fyz;
+      b.fxz = p->fxz;
+      b.vo32 = nullptr;
+      b.vi32 = nullptr;
+      b.items = p->items;
+      b.nx = d.ncell_x;
+      b.ny = d.ncell_y;
+      b.nz = p->zb;
+      b.ne = d.ne;
+      b.na = d.na;
+      b.nu = d.nu;
+      b.na_pad = p->na_pad;
+      b.ntx = p->ntx;
+      b.nty = p->nty;
+      b.z_off = s.k * p->zb;
+      b.nz_grid = d.ncell_z;
+      b.fz = p->fz;
+      b.z_in = s.kk > 0;
+      b.z_out = s.kk < p->nblk - 1;
+      launch_mfma_fill(p->precision, b, st);
+      rc = launch_range(p, b, st, s.pass == 0 ? 0 : p->pass_split,
+                        s.pass == 0 ? p->pass_split : p->launch_off.size());
+    }
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(p->ev_done[sl], st));
+    HIP_TRY(hipStreamWaitEvent(p->cst, p->ev_done[sl], 0));
+    HIP_TRY(hipMemcpyAsync(vo_host + (size_t)s.k * slab, p->slab_vo[sl], bytes,
+                           hipMemcpyDeviceToHost, p->cst));
+    if (turn && (rc = copy_in(i + 1))) return rc;
+  }
+  // the caller's stream covers the last copy back
+  HIP_TRY(hipEventRecord(p->ev_in[0], p->cst));
+  HIP_TRY(hipStreamWaitEvent(st, p->ev_in[0], 0));
+  return MSW_OK;
 }
 
 int msw_state_norms(const MswDims* d, const double* vi, const double* vo,

@@ -244,3 +244,24 @@ def test_server_tuned_from_batch(cuda):
     srv.close()
     b.close()
     b64.close()
+
+
+def test_batch_more_problems_than_one_launch_holds(cuda):
+    """More problems of one kernel variant than one combined launch takes
+    (128 segments): each tick splits them over several launches; every
+    result is exact (manufactured input) and equal to its single sweep."""
+    import torch
+    print("case batch 200 problems of one variant", flush=True)
+    shapes = [(5 + i % 4, 3 + i % 3, 2 + i % 2, 1, 4, 32, 1) for i in range(200)]
+    ins = _inputs(torch, cuda, shapes, perturb=False)
+    b = msw.Batch([x[0] for x in ins], precision=msw.PREC_FP64)
+    for i, (d, vi0, vi, vo, a, m) in enumerate(ins):
+        b.set_problem(i, vi, vo, a, m)
+    b.run()
+    torch.cuda.synchronize()
+    for (d, vi0, vi, vo, a, m), s in zip(ins, shapes):
+        n, nd = msw.state_norms(d, vi, vo)
+        assert nd == 0.0 and n == float(closed_form_normsq(s[0], s[1], s[2], s[3], s[4], s[6]))
+    d, vi0, vi, vo, a, m = ins[7]
+    np.testing.assert_array_equal(vo.cpu().numpy(), _single(torch, d, vi, a, m, msw.PREC_FP64))
+    b.close()
