@@ -109,8 +109,8 @@ int msw_plan_sweep(MswPlan* plan, const double* vi, double* vo,
  * in the state layout, pinned (hipHostMalloc) for the copies to overlap the
  * sweep.  The state goes through HBM one z block at a time, each block's vi
  * (and, in the second pass, its partial vo) copied in while the previous
- * block is swept, its vo copied back after: two passes of four octants, 6
- * state volumes over PCIe per sweep.  Device memory: two slabs of vi and vo
+ * block is swept, its vo copied back after: two passes of four octants, 5
+ * state volumes over PCIe per sweep (vi twice, vo back twice, vo in once).  Device memory: two slabs of vi and vo
  * of one block each, plus the plan.  a_from_m, m_from_a are device arrays.
  * Enqueued on stream; synchronize it before reading vo_host. */
 int msw_plan_sweep_host(MswPlan* plan, const double* vi_host, double* vo_host,

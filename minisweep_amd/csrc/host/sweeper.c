@@ -150,7 +150,14 @@ void Sweeper_create(Sweeper* sweeper, Dimensions dims, const Quantities* quan,
     d.na = dims.na;
     d.nu = NU;
     sweeper->plan = NULL;
-    MSW_INSIST_RC(msw_plan_create(&d, sweeper->precision, &sweeper->plan));
+    /* --nblock_z: the FP64 sweep runs z block by z block, as the reference's
+       (sweeper_kba_c.h:719-781); the mixed forms sweep the grid at once */
+    MswPlanOptions opt;
+    msw_plan_default_options(&opt);
+    opt.precision = sweeper->precision;
+    if (sweeper->nblock_z > 1 && sweeper->precision == MSW_PREC_FP64)
+      opt.z_block = dims.ncell_z / sweeper->nblock_z;
+    MSW_INSIST_RC(msw_plan_create_ex(&d, &opt, &sweeper->plan));
   }
 }
 
