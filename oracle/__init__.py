@@ -40,6 +40,7 @@ def lib():
         L.oracle_init_matrices.argtypes = [OracleDims, dp, dp]
         L.oracle_init_state.argtypes = [OracleDims, dp]
         L.oracle_sweep.argtypes = [OracleDims, dp, dp, dp, dp, ctypes.c_int]
+        L.oracle_sweep_tileoctants.argtypes = [OracleDims, dp, dp, dp, dp, ctypes.c_int]
         L.oracle_sweep_range.argtypes = [OracleDims, dp, dp, dp, dp,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.oracle_norms.argtypes = [OracleDims, dp, dp, dp, dp]
@@ -83,6 +84,16 @@ def sweep(d: OracleDims, a, m, vi, nthreads: int = 0) -> np.ndarray:
     vo = np.zeros_like(vi)
     lib().oracle_sweep(d, _p(np.ascontiguousarray(a)), _p(np.ascontiguousarray(m)),
                        _p(vi), _p(vo), nthreads)
+    return vo
+
+
+def sweep_tileoctants(d: OracleDims, a, m, vi, nthreads: int = 0) -> np.ndarray:
+    """The sweep in the reference's tiled-octant order (8 disjoint
+    half-domain boxes per step); equal to sweep() up to rounding."""
+    vi = np.ascontiguousarray(vi, dtype=np.float64)
+    vo = np.zeros_like(vi)
+    lib().oracle_sweep_tileoctants(d, _p(np.ascontiguousarray(a)), _p(np.ascontiguousarray(m)),
+                                   _p(vi), _p(vo), nthreads)
     return vo
 
 

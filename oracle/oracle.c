@@ -145,22 +145,20 @@ double oracle_face_value(OracleDims d, int ix_g, int iy_g, int iz_g, int ie,
          (double)sf_unknown(iu) * (double)sf_octant(octant);
 }
 
-/* Per-(octant, ie) sweep over the whole grid; faces are private to the call.
- * Body follows sweeper_simple_c.h:118-270 with the solve of
- * quantities_testing_kernels.h:380-446. */
-static void sweep_one(const OracleDims* d, const double* A, const double* M,
-                      const double* vi, double* vo_part, int octant, int ie,
-                      double* fxy, double* fxz, double* fyz, double* vs) {
-  const int nx = d->ncell_x, ny = d->ncell_y, nz = d->ncell_z;
-  const int na = d->na, nm = d->nm, nu = d->nu;
-  const int dx = dir_of(octant, 0), dy = dir_of(octant, 1), dz = dir_of(octant, 2);
-  int ix, iy, iz, ia, iu, im;
-  /* face layouts as ref_facexy/xz/yz with one octant and one ie:
-   * fxy[iy][ix][iu][ia], fxz[iz][ix][iu][ia], fyz[iz][iy][iu][ia] */
+/* face layouts as ref_facexy/xz/yz with one octant and one ie:
+ * fxy[iy][ix][iu][ia], fxz[iz][ix][iu][ia], fyz[iz][iy][iu][ia] */
 #define FXY(x, y, a, u) fxy[(a) + na * ((u) + nu * ((x) + nx * (y)))]
 #define FXZ(x, z, a, u) fxz[(a) + na * ((u) + nu * ((x) + nx * (z)))]
 #define FYZ(y, z, a, u) fyz[(a) + na * ((u) + nu * ((y) + ny * (z)))]
-  /* face init "one cell outside" the domain (sweeper_simple_c.h:124-186) */
+
+/* Faces of one (octant, ie) set "one cell outside" the domain
+ * (sweeper_simple_c.h:124-186). */
+static void init_faces(const OracleDims* d, int octant, int ie, double* fxy, double* fxz,
+                       double* fyz) {
+  const int nx = d->ncell_x, ny = d->ncell_y, nz = d->ncell_z;
+  const int na = d->na, nu = d->nu;
+  const int dx = dir_of(octant, 0), dy = dir_of(octant, 1), dz = dir_of(octant, 2);
+  int ix, iy, iz, ia, iu;
   iz = dz == +1 ? -1 : nz;
   for (iy = 0; iy < ny; ++iy)
     for (ix = 0; ix < nx; ++ix)
@@ -179,16 +177,30 @@ static void sweep_one(const OracleDims* d, const double* A, const double* M,
       for (iu = 0; iu < nu; ++iu)
         for (ia = 0; ia < na; ++ia)
           FYZ(iy, iz, ia, iu) = oracle_face_value(*d, ix, iy, iz, ie, ia, iu, octant);
+}
 
+/* One (octant, ie) over the cells [x0, x1) x [y0, y1) x [z0, z1), upwind
+ * first; the faces carry the values "one cell upwind" in and out, so a box
+ * continues where the previous boxes of the octant left them.  Body follows
+ * sweeper_simple_c.h:187-270 with the solve of
+ * quantities_testing_kernels.h:380-446. */
+static void sweep_box(const OracleDims* d, const double* A, const double* M,
+                      const double* vi, double* vo_part, int octant, int ie,
+                      double* fxy, double* fxz, double* fyz, double* vs, int x0, int x1,
+                      int y0, int y1, int z0, int z1) {
+  const int nx = d->ncell_x, ny = d->ncell_y;
+  const int na = d->na, nm = d->nm, nu = d->nu;
+  const int dx = dir_of(octant, 0), dy = dir_of(octant, 1), dz = dir_of(octant, 2);
+  int ix, iy, iz, ia, iu, im;
   {
-    const int ixb = dx == +1 ? 0 : nx - 1, iyb = dy == +1 ? 0 : ny - 1,
-              izb = dz == +1 ? 0 : nz - 1;
+    const int ixb = dx == +1 ? x0 : x1 - 1, iyb = dy == +1 ? y0 : y1 - 1,
+              izb = dz == +1 ? z0 : z1 - 1;
     const double so = (double)sf_octant(octant);
     const double so_r = 1.0 / so;
     int cz, cy, cx;
-    for (cz = 0, iz = izb; cz < nz; ++cz, iz += dz)
-      for (cy = 0, iy = iyb; cy < ny; ++cy, iy += dy)
-        for (cx = 0, ix = ixb; cx < nx; ++cx, ix += dx) {
+    for (cz = 0, iz = izb; cz < z1 - z0; ++cz, iz += dz)
+      for (cy = 0, iy = iyb; cy < y1 - y0; ++cy, iy += dy)
+        for (cx = 0, ix = ixb; cx < x1 - x0; ++cx, ix += dx) {
           const size_t base = ind_state(d, ix, iy, iz, ie, 0, 0);
           const double s = (double)sf_space(ix, iy, iz);
           const double s_r = 1.0 / s;
@@ -230,9 +242,18 @@ static void sweep_one(const OracleDims* d, const double* A, const double* M,
             }
         }
   }
+}
 #undef FXY
 #undef FXZ
 #undef FYZ
+
+/* Per-(octant, ie) sweep over the whole grid; faces are private to the call. */
+static void sweep_one(const OracleDims* d, const double* A, const double* M,
+                      const double* vi, double* vo_part, int octant, int ie,
+                      double* fxy, double* fxz, double* fyz, double* vs) {
+  init_faces(d, octant, ie, fxy, fxz, fyz);
+  sweep_box(d, A, M, vi, vo_part, octant, ie, fxy, fxz, fyz, vs, 0, d->ncell_x, 0,
+            d->ncell_y, 0, d->ncell_z);
 }
 
 void oracle_sweep_range(OracleDims d, const double* A, const double* M,
@@ -277,6 +298,52 @@ void oracle_sweep_range(OracleDims d, const double* A, const double* M,
 void oracle_sweep(OracleDims d, const double* A, const double* M,
                   const double* vi, double* vo, int nthreads) {
   oracle_sweep_range(d, A, M, vi, vo, 0, d.ne, nthreads);
+}
+
+/* The reference's tiled-octant ordering (sweeper_tileoctants_c.h:127-189):
+ * 8 tile steps; at step t octant o sweeps the half-domain box whose half
+ * along axis b is the lower one when (bit b of t is 0) equals (o runs upward
+ * along b), so the 8 octants of a step work on disjoint boxes and each
+ * octant visits its boxes upwind first.  Each octant keeps its faces between
+ * steps.  The cells receive the octants' contributions in another order than
+ * oracle_sweep's 0..7, so the two agree to rounding. */
+void oracle_sweep_tileoctants(OracleDims d, const double* A, const double* M,
+                              const double* vi, double* vo, int nthreads) {
+  const int nx = d.ncell_x, ny = d.ncell_y, nz = d.ncell_z;
+  const size_t nfxy = (size_t)nx * ny * d.na * d.nu;
+  const size_t nfxz = (size_t)nx * nz * d.na * d.nu;
+  const size_t nfyz = (size_t)ny * nz * d.na * d.nu;
+  const size_t nf = nfxy + nfxz + nfyz;
+  int ie;
+  memset(vo, 0, sizeof(double) * oracle_size_state(d));
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+  for (ie = 0; ie < d.ne; ++ie) {
+    double* faces = (double*)malloc(sizeof(double) * nf * NOCT);
+    double* vs = (double*)malloc(sizeof(double) * (size_t)d.na * d.nu);
+    int step, octant;
+    for (octant = 0; octant < NOCT; ++octant) {
+      double* f = faces + nf * octant;
+      init_faces(&d, octant, ie, f, f + nfxy, f + nfxy + nfxz);
+    }
+    for (step = 0; step < NOCT; ++step)
+      for (octant = 0; octant < NOCT; ++octant) {
+        double* f = faces + nf * octant;
+        int lo[3], hi[3], ext[3] = {nx, ny, nz}, b;
+        for (b = 0; b < 3; ++b) {
+          const int lower = (((step >> b) & 1) == 0) == (dir_of(octant, b) == +1);
+          lo[b] = lower ? 0 : ext[b] / 2;
+          hi[b] = lower ? ext[b] / 2 : ext[b];
+        }
+        sweep_box(&d, A, M, vi, vo, octant, ie, f, f + nfxy, f + nfxy + nfxz, vs, lo[0], hi[0],
+                  lo[1], hi[1], lo[2], hi[2]);
+      }
+    free(faces);
+    free(vs);
+  }
+  (void)nthreads;
 }
 
 /* get_state_norms (array_operations.c:69-110), same loop order */

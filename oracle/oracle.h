@@ -50,6 +50,12 @@ void oracle_init_state(OracleDims d, double* v);
 void oracle_sweep(OracleDims d, const double* a_from_m, const double* m_from_a,
                   const double* vi, double* vo, int nthreads);
 
+/* The same sweep in the reference's tiled-octant order
+ * (sweeper_tileoctants_c.h:127-189): a second CPU ordering, equal to
+ * oracle_sweep up to rounding. */
+void oracle_sweep_tileoctants(OracleDims d, const double* a_from_m, const double* m_from_a,
+                              const double* vi, double* vo, int nthreads);
+
 /* Same sweep restricted to the energy groups [ie_begin, ie_end): used to time
  * a bounded sample of a large problem. */
 void oracle_sweep_range(OracleDims d, const double* a_from_m,
