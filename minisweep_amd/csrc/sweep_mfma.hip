@@ -236,8 +236,9 @@ struct Occupancy {
 // [ie0, ie0 + nge) streamed back to back through one z pipeline, unknown iu,
 // angle chunk = wave index.  Lane (j, b) is at stream position
 // t - j - b = (group - ie0) * nz + z, so the tile's fill and drain are paid
-// once per nge groups.
-template <class PR, int NM, int NAC, int NWA, int NY>
+// once per nge groups.  ZB: built for z blocks (z faces between blocks);
+// the whole-grid kernels carry none of that code.
+template <class PR, int NM, int NAC, int NWA, int NY, bool ZB>
 __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem it,
                                            const int ie0, const int nge, const int iu) {
   typedef typename PR::T T;
@@ -547,7 +548,7 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
 #pragma unroll
         for (int r = 0; r < 4; ++r) rz[r] = F[a][b][r];
         if (any_ent[b]) {  // some lane of this block starts a group
-          if (p.z_in) {  // inflow: the previous z block's last plane
+          if (ZB && p.z_in) {  // inflow: the previous z block's last plane
             T zi[4];
             load4(fz_cell(b) + 16 * a, zi);
 #pragma unroll
@@ -619,7 +620,7 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
           for (int a = 0; a < NAB; ++a) store4(dst + 16 * a, F[a][b]);
         }
     }
-    if (p.z_out) {  // the block's last plane is the next block's z inflow
+    if (ZB && p.z_out) {  // the block's last plane is the next block's z inflow
 #pragma unroll
       for (int b = 0; b < kTY; ++b)
         if (val[b] && zs[b] == nz - 1) {
@@ -698,7 +699,7 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
 }
 
 // Workgroup bid of a launch -> (item, chunk of epw energy groups, unknown).
-template <class PR, int NM, int NAC, int NWA, int NY>
+template <class PR, int NM, int NAC, int NWA, int NY, bool ZB = false>
 __device__ __forceinline__ void sweep_block(const SweepParams& p, int item_base, int bid,
                                             int epw) {
   const int iu = bid % p.nu;
@@ -707,15 +708,16 @@ __device__ __forceinline__ void sweep_block(const SweepParams& p, int item_base,
   const int ec = bid % nec;
   bid /= nec;
   const int ie0 = ec * epw;
-  sweep_tile<PR, NM, NAC, NWA, NY>(p, p.items[item_base + bid], ie0, min(epw, p.ne - ie0), iu);
+  sweep_tile<PR, NM, NAC, NWA, NY, ZB>(p, p.items[item_base + bid], ie0, min(epw, p.ne - ie0),
+                                       iu);
 }
 
-// Single sweep.
-template <class PR, int NM, int NAC, int NWA, int NY>
+// Single sweep (ZB: one z block of it).
+template <class PR, int NM, int NAC, int NWA, int NY, bool ZB>
 __global__ __launch_bounds__(64 * NWA * NY) __attribute__((amdgpu_waves_per_eu(
     Occupancy<PR, NM, NAC, NWA, NY>::value))) void k_sweep_mfma(SweepParams p, int item_base,
                                                                 int epw) {
-  sweep_block<PR, NM, NAC, NWA, NY>(p, item_base, (int)blockIdx.x, epw);
+  sweep_block<PR, NM, NAC, NWA, NY, ZB>(p, item_base, (int)blockIdx.x, epw);
 }
 
 // Many sweeps in one launch: segment k of the launch is one launch of the
@@ -806,8 +808,15 @@ static void launch_nac(const LaunchSpec& ls, const SweepParams& sp, hipStream_t 
   const dim3 block(64 * ls.nwa * ls.ny);
 #define MSW_CASE(W, Y)                                                                     \
   if (ls.nwa == W && ls.ny == Y) {                                                         \
-    hipLaunchKernelGGL((k_sweep_mfma<PR, NM, NAC, W, Y>), dim3(ls.grid), block, ls.lds_bytes, \
-                       st, sp, ls.item_base, ls.epw);                                     \
+    if constexpr (std::is_same<PR, PrecF64>::value) {                                     \
+      if (sp.fz) {                                                                         \
+        hipLaunchKernelGGL((k_sweep_mfma<PR, NM, NAC, W, Y, true>), dim3(ls.grid), block,   \
+                           ls.lds_bytes, st, sp, ls.item_base, ls.epw);                   \
+        return;                                                                            \
+      }                                                                                    \
+    }                                                                                      \
+    hipLaunchKernelGGL((k_sweep_mfma<PR, NM, NAC, W, Y, false>), dim3(ls.grid), block,      \
+                       ls.lds_bytes, st, sp, ls.item_base, ls.epw);                       \
     return;                                                                                \
   }
   MSW_SHAPES(MSW_CASE)
