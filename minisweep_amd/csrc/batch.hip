@@ -135,14 +135,16 @@ int make_geom(const MswDims& d, int nac, Geom* g, int ny = 0, int precision = MS
 }
 
 // FP32 running vo sum of the mixed forms (none in FP64), followed in the
-// BF16 form by the FP32 copy of vi.
+// BF16 form by the BF16 copy of vi.
 size_t vo32_bytes(const Geom& g, int precision) {
   const size_t one = align_up(sizeof(float) * g.state_doubles());
-  return precision == MSW_PREC_FP64 ? 0 : precision == MSW_PREC_MIXED_BF16 ? 2 * one : one;
+  return precision == MSW_PREC_FP64         ? 0
+         : precision == MSW_PREC_MIXED_BF16 ? one + align_up(sizeof(short) * g.state_doubles())
+                                            : one;
 }
-const float* vi32_of(const Geom& g, float* vo32, int precision) {
+const unsigned short* vib_of(const Geom& g, float* vo32, int precision) {
   return vo32 && precision == MSW_PREC_MIXED_BF16
-             ? (const float*)((char*)vo32 + align_up(sizeof(float) * g.state_doubles()))
+             ? (const unsigned short*)((char*)vo32 + align_up(sizeof(float) * g.state_doubles()))
              : nullptr;
 }
 
@@ -151,7 +153,7 @@ SweepParams make_params(const Geom& g, const double* vi, double* vo, const doubl
                         const DevSchedule* s, int precision) {
   SweepParams sp;
   sp.vo32 = vo32;
-  sp.vi32 = vi32_of(g, vo32, precision);
+  sp.vib = vib_of(g, vo32, precision);
   sp.vi = vi;
   sp.vo = vo;
   sp.at = at;
@@ -389,8 +391,8 @@ int batch_enqueue(MswBatch* b, hipStream_t st) {
   for (auto& p : b->probs) {
     launch_pack(p.afm, p.mfa, p.at, p.mf, p.g.d.nm, p.g.d.na, p.g.nap, st);
     ++nk;
-    if (const float* v32 = vi32_of(p.g, p.vo32, b->opt.precision)) {
-      launch_to_f32(p.vi, (float*)v32, p.g.state_doubles(), st);
+    if (const unsigned short* vb = vib_of(p.g, p.vo32, b->opt.precision)) {
+      launch_to_bf16(p.vi, (unsigned short*)vb, p.g.state_doubles(), st);
       ++nk;
     }
   }
@@ -1035,8 +1037,8 @@ void server_loop(MswServer* s) {
                                hipMemcpyHostToDevice, s->st);
         else
           launch_init_state(a.g.d, a.vi, s->st);
-        if (const float* v32 = vi32_of(a.g, a.vo32, s->opt.precision))
-          launch_to_f32(a.vi, (float*)v32, a.g.state_doubles(), s->st);
+        if (const unsigned short* vb = vib_of(a.g, a.vo32, s->opt.precision))
+          launch_to_bf16(a.vi, (unsigned short*)vb, a.g.state_doubles(), s->st);
         fill.emplace_back(a.slot, &s->h_params[a.slot]);
         a.started = true;
       }

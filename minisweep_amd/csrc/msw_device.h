@@ -73,7 +73,8 @@ struct SweepParams {
                      //   MFMA [8][nu][ny][ne][nz][na] (octant-frame z)
   void* fxz;         // y-direction faces: the same with nx for ny
   float* vo32;       // mixed forms: FP32 running sum of vo over the octants (state layout)
-  const float* vi32; // mixed forms: vi rounded to FP32 once per sweep (state layout)
+  const unsigned short* vib;  // BF16 form: vi rounded to BF16 (through FP32) once per
+                              // sweep, state layout
   const TileItem* items;
   int nx, ny, nz, ne, na, nu;
   int na_pad;  // na rounded up to whole angle chunks (matrix/face stride)

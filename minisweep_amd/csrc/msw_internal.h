@@ -78,7 +78,8 @@ void launch_pack(const double* afm, const double* mfa, double* at, double* mf, i
                  int na_pad, hipStream_t st);
 void launch_init_state(const MswDims& d, double* v, hipStream_t st);
 size_t norms_scratch_doubles();
-void launch_to_f32(const double* src, float* dst, size_t n, hipStream_t st);
+// dst <- BF16 bits of (float)src, 4 values per thread; n a multiple of 4.
+void launch_to_bf16(const double* src, unsigned short* dst, size_t n, hipStream_t st);
 void launch_norms(const double* vi, const double* vo, size_t n, double* part, double* out2,
                   hipStream_t st);
 

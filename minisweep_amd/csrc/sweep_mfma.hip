@@ -156,6 +156,7 @@ __device__ __forceinline__ void store4(float* d, const float (&v)[4]) {
 struct PrecF64 {
   typedef double T;
   typedef double A;
+  typedef double Vin;  // m2a B operand as held in registers
   typedef d4v acc4;
   static constexpr bool kPacked = false;
   static constexpr bool kPingPong = false;
@@ -169,6 +170,7 @@ struct PrecF64 {
 struct PrecF32 {
   typedef float T;
   typedef float A;
+  typedef float Vin;
   typedef f4v acc4;
   static constexpr bool kPacked = false;
   static constexpr bool kPingPong = true;
@@ -182,6 +184,7 @@ struct PrecF32 {
 struct PrecBF16 {
   typedef float T;
   typedef float A;
+  typedef short Vin;  // the BF16 bits of vi, read from the BF16 copy
   typedef f4v acc4;
   static constexpr bool kPacked = true;  // K = 16: 4 values per lane per MFMA
   static constexpr bool kPingPong = true;
@@ -194,6 +197,16 @@ struct PrecBF16 {
     for (int e = 0; e < 4; ++e) {
       pa[e] = __builtin_bit_cast(short, (__bf16)a[e]);
       pb[e] = __builtin_bit_cast(short, (__bf16)b[e]);
+    }
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(pa, pb, c, 0, 0, 0);
+  }
+  // the same with B already in BF16
+  static __device__ __forceinline__ acc4 mma16b(const T (&a)[4], const short (&b)[4], acc4 c) {
+    s4v pa, pb;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      pa[e] = __builtin_bit_cast(short, (__bf16)a[e]);
+      pb[e] = b[e];
     }
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(pa, pb, c, 0, 0, 0);
   }
@@ -374,7 +387,8 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
   // agree across paths and each wait covers only the data it needs. ---------
   // m2a B operand, two buffers: one step's while the next step's loads are in
   // flight (a whole step ahead)
-  T vbA[kTY][KA], vbB[kTY][KA];
+  typedef typename PR::Vin Vin;
+  Vin vbA[kTY][KA], vbB[kTY][KA];
   // running vo sum of the owned blocks' cells from earlier octant visits:
   // FP64 vo in the FP64 form, the FP32 sum in the mixed forms
   typedef typename std::conditional<kF64, double, float>::type V;
@@ -431,19 +445,25 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
     const int y = dy > 0 ? gy * kTYT + bc : gy * kTYT + wy - 1 - bc;
     xs_src = fyz + y * fplane + (j & 3);
   }
-  // the input state: FP64, or its FP32 copy in the BF16 form
-  typedef typename std::conditional<PR::kPacked, float, double>::type VI;
-  const VI* vsrc = PR::kPacked ? (const VI*)p.vi32 : (const VI*)p.vi;
-  auto fetch_vi = [&](const size_t (&so)[kTY], T (&vnx)[kTY][KA]) {
+  // the input state: FP64, or its BF16 copy in the BF16 form (4 values of a
+  // lane in 8 bytes)
+  auto fetch_vi = [&](const size_t (&so)[kTY], Vin (&vnx)[kTY][KA]) {
 #pragma unroll
     for (int b = 0; b < kTY; ++b) {
-      const VI* src = vsrc + so[b] + moff;
-      if constexpr (KA == 4) {
-        VI v[4] = {0, 0, 0, 0};
-        if (!mpad) load4(src, v);
+      if constexpr (PR::kPacked) {
+        uint2 w = {0u, 0u};
+        if (!mpad) w = *reinterpret_cast<const uint2*>(p.vib + so[b] + moff);
+        vnx[b][0] = (short)(w.x & 0xffffu);
+        vnx[b][1] = (short)(w.x >> 16);
+        vnx[b][2] = (short)(w.y & 0xffffu);
+        vnx[b][3] = (short)(w.y >> 16);
+      } else if constexpr (KA == 4) {
+        double v[4] = {0, 0, 0, 0};
+        if (!mpad) load4(p.vi + so[b] + moff, v);
 #pragma unroll
         for (int s = 0; s < 4; ++s) vnx[b][s] = (T)v[s];
       } else {
+        const double* src = p.vi + so[b] + moff;
 #pragma unroll
         for (int s = 0; s < KA; ++s) vnx[b][s] = (T)src[s];
       }
@@ -475,8 +495,8 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
 
   // One step: this step's operands are vin and xsc, the next step's are
   // fetched into vnx and xsn first thing.
-  auto step = [&](const int t, const T (&vin)[kTY][KA], T (&vnx)[kTY][KA], const T (&xsc)[NAB],
-                  T (&xsn)[NAB]) {
+  auto step = [&](const int t, const Vin (&vin)[kTY][KA], Vin (&vnx)[kTY][KA],
+                  const T (&xsc)[NAB], T (&xsn)[NAB]) {
     advance();
     fetch_vi(sn, vnx);
     bump(pxs, oxs);
@@ -512,7 +532,7 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
       for (int b = kTY - 1; b >= 0; --b) {
         vs[b] = acc4{0, 0, 0, 0};
         if constexpr (PR::kPacked) {
-          vs[b] = PR::mma16(aop[a], vin[b], vs[b]);
+          vs[b] = PR::mma16b(aop[a], vin[b], vs[b]);
         } else {
 #pragma unroll
           for (int s = 0; s < KM; ++s) vs[b] = PR::mma(aop[a][s], vin[b][s], vs[b]);
@@ -761,16 +781,35 @@ __device__ __forceinline__ void fill_row(const SweepParams& p, int row) {
   T* dst = xface ? reinterpret_cast<T*>(p.fyz) + (size_t)row * len
                  : reinterpret_cast<T*>(p.fxz) + (size_t)(row - nrow_x) * len;
   const int su = sf_unknown(iu);
-  for (int i = threadIdx.x; i < (int)len; i += blockDim.x) {
-    const int pos = i % nap, q = i / nap;
+  // boundary cell coordinates outside the grid: x (x faces) or y (y faces)
+  const int bx = xface ? ((oct & 1) ? p.nx : -1) : c;
+  const int by = xface ? c : ((oct & 2) ? p.ny : -1);
+  // Lanes cover runs of 4 angles (nap is a multiple of 16): with nap/4 <= 64
+  // runs per (group, z) row a wave takes 64 / (nap/4) rows per pass, lane
+  // (row qo, run r0); wider rows take one row per pass.  The row's scale
+  // sf_energy * sf_unknown * 2^space is a power of two, so (1 + ia) 2^(ia & 7)
+  // times it is exact, the value Quantities_init_face gives.
+  const int nap4 = nap >> 2, nq = p.ne * nz;
+  const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+  const int nwave = (int)blockDim.x >> 6;
+  const int per = nap4 <= 64 ? 64 / nap4 : 1;
+  const int qo = nap4 <= 64 ? lane / nap4 : 0;
+  const int r0 = nap4 <= 64 ? lane % nap4 : lane, rstep = nap4 <= 64 ? nap4 : 64;
+  if (qo >= per) return;  // lanes past the last whole row of a pass
+  for (int q = wave * per + qo; q < nq; q += nwave * per) {
     const int zq = q % nz, ie = q / nz;
     const int z = p.z_off + (dz > 0 ? zq : nz - 1 - zq);  // grid plane
-    const int ia = 16 * (pos >> 4) + PR::row((pos & 15) >> 2, pos & 3);
-    const int e = xface ? sf_space_exp((oct & 1) ? p.nx : -1, c, z)
-                        : sf_space_exp(c, (oct & 2) ? p.ny : -1, z);
-    dst[i] = ia < p.na
-                 ? (T)ldexp((double)((1 + ia) << (ia & 7)) * (double)(sf_energy(ie) * su), e)
-                 : (T)0;
+    const double sc = ldexp((double)(sf_energy(ie) * su), sf_space_exp(bx, by, z));
+    T* row_dst = dst + (size_t)q * nap;
+    for (int r4 = r0; r4 < nap4; r4 += rstep) {
+      T v[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int ia = 16 * (r4 >> 2) + PR::row(r4 & 3, r);
+        v[r] = ia < p.na ? (T)((double)((1 + ia) << (ia & 7)) * sc) : (T)0;
+      }
+      store4(row_dst + 4 * r4, v);
+    }
   }
 }
 

@@ -372,15 +372,21 @@ __global__ void k_pack_matrices(const double* __restrict__ afm,
 }
 
 // Manufactured state v = (1+im) S(x,y,z) E(ie) U(iu), layout [iz][ie][iy][ix][iu][im].
-// FP64 -> FP32 copy of a state (the BF16 form's input, once per sweep), 4
-// values per thread and step; n is a multiple of 4 (nm is).
-__global__ __launch_bounds__(256) void k_state_to_f32(const double* __restrict__ src,
-                                                      float* __restrict__ dst, size_t n) {
+// FP64 -> BF16 copy of a state (the BF16 form's input, once per sweep): each
+// value rounded to FP32, then to BF16 (nearest even), the two roundings the
+// form's contractions apply; 4 values per thread and step, n a multiple of 4.
+__global__ __launch_bounds__(256) void k_state_to_bf16(const double* __restrict__ src,
+                                                       unsigned short* __restrict__ dst,
+                                                       size_t n) {
   for (size_t i = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) * 4; i < n;
        i += (size_t)gridDim.x * blockDim.x * 4) {
     const double2 a = *reinterpret_cast<const double2*>(src + i);
     const double2 b = *reinterpret_cast<const double2*>(src + i + 2);
-    *reinterpret_cast<float4*>(dst + i) = float4{(float)a.x, (float)a.y, (float)b.x, (float)b.y};
+    const unsigned short h0 = __builtin_bit_cast(unsigned short, (__bf16)(float)a.x);
+    const unsigned short h1 = __builtin_bit_cast(unsigned short, (__bf16)(float)a.y);
+    const unsigned short h2 = __builtin_bit_cast(unsigned short, (__bf16)(float)b.x);
+    const unsigned short h3 = __builtin_bit_cast(unsigned short, (__bf16)(float)b.y);
+    *reinterpret_cast<uint2*>(dst + i) = uint2{h0 | (unsigned)h1 << 16, h2 | (unsigned)h3 << 16};
   }
 }
 
@@ -544,9 +550,9 @@ void launch_init_state(const MswDims& d, double* v, hipStream_t st) {
 
 size_t norms_scratch_doubles() { return 2 * (size_t)kNormBlocks; }
 
-void launch_to_f32(const double* src, float* dst, size_t n, hipStream_t st) {
+void launch_to_bf16(const double* src, unsigned short* dst, size_t n, hipStream_t st) {
   const unsigned blocks = (unsigned)std::min<size_t>((n / 4 + 255) / 256, 8192);
-  hipLaunchKernelGGL(k_state_to_f32, dim3(blocks), dim3(256), 0, st, src, dst, n);
+  hipLaunchKernelGGL(k_state_to_bf16, dim3(blocks), dim3(256), 0, st, src, dst, n);
 }
 
 void launch_norms(const double* vi, const double* vo, size_t n, double* part, double* out2,
@@ -573,7 +579,7 @@ struct MswPlan {
   void* fyz;   // face planes (element type of the precision)
   void* fxz;
   float* vo32;  // mixed forms: FP32 running vo sum over the octants
-  float* vi32;  // mixed forms: FP32 copy of vi
+  unsigned short* vib;  // BF16 form: BF16 copy of vi
   double* norm_part;
   TileItem* items;
   std::vector<int> launch_off, launch_cnt;
@@ -835,8 +841,8 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
     // the BF16 form reads an FP32 copy of vi (measured faster at 64^3); the
     // FP32 form reads vi itself, the copy would cost more than it saves
     if (e == hipSuccess && precision == MSW_PREC_MIXED_BF16) {
-      e = hipMalloc(&p->vi32, sizeof(float) * msw_size_state(d));
-      p->bytes += sizeof(float) * msw_size_state(d);
+      e = hipMalloc(&p->vib, sizeof(unsigned short) * msw_size_state(d));
+      p->bytes += sizeof(unsigned short) * msw_size_state(d);
     }
   }
   if (e == hipSuccess) e = hipMalloc(&p->norm_part, sizeof(double) * 2 * kNormBlocks);
@@ -901,7 +907,7 @@ int msw_plan_destroy(MswPlan* p) {
   if (p->fyz) (void)hipFree(p->fyz);
   if (p->fxz) (void)hipFree(p->fxz);
   if (p->vo32) (void)hipFree(p->vo32);
-  if (p->vi32) (void)hipFree(p->vi32);
+  if (p->vib) (void)hipFree(p->vib);
   if (p->norm_part) (void)hipFree(p->norm_part);
   if (p->items) (void)hipFree(p->items);
   if (p->fz) (void)hipFree(p->fz);
@@ -931,7 +937,7 @@ static int sweep_device(MswPlan* p, const double* vi, double* vo, hipStream_t st
   sp.fyz = p->fyz;
   sp.fxz = p->fxz;
   sp.vo32 = p->vo32;
-  sp.vi32 = p->vi32;
+  sp.vib = p->vib;
   sp.items = p->items;
   sp.nx = d.ncell_x;
   sp.ny = d.ncell_y;
@@ -943,7 +949,7 @@ static int sweep_device(MswPlan* p, const double* vi, double* vo, hipStream_t st
   sp.ntx = p->ntx;
   sp.nty = p->nty;
   if (p->nblk == 1) {
-    if (p->vi32) launch_to_f32(vi, p->vi32, msw_size_state(&d), st);
+    if (p->vib) launch_to_bf16(vi, p->vib, msw_size_state(&d), st);
     if (p->engine == MSW_ENGINE_MFMA) launch_mfma_fill(p->precision, sp, st);
     return launch_range(p, sp, st, 0, p->launch_off.size());
   }
@@ -1049,7 +1055,7 @@ int msw_plan_sweep_host(MswPlan* p, const double* vi_host, double* vo_host,
       b.fyz = p->fyz;
       b.fxz = p->fxz;
       b.vo32 = nullptr;
-      b.vi32 = nullptr;
+      b.vib = nullptr;
       b.items = p->items;
       b.nx = d.ncell_x;
       b.ny = d.ncell_y;
