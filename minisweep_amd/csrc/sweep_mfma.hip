@@ -488,10 +488,31 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
     for (int a = 0; a < NAB; ++a) load4(fxz + oyi + 16 * a, yinn[a]);
   };
 
+  // m2a of angle block a: VS[angle][cell] = AT . VI
+  auto m2a = [&](int a, const Vin (&vin)[kTY][KA], acc4 (&vs)[kTY]) {
+#pragma unroll
+    for (int b = kTY - 1; b >= 0; --b) {
+      vs[b] = acc4{0, 0, 0, 0};
+      if constexpr (PR::kPacked) {
+        vs[b] = PR::mma16b(aop[a], vin[b], vs[b]);
+      } else {
+#pragma unroll
+        for (int s = 0; s < KM; ++s) vs[b] = PR::mma(aop[a][s], vin[b][s], vs[b]);
+      }
+    }
+  };
+
   fetch_xs(PR::kPingPong ? xbA : xbB);
   fetch_vi(sc, PR::kPingPong ? vbA : vbB);
   if (wyi == 0) fetch_yin();
   fetch_vo(sc);
+  // BF16: m2a of angle block 0 for the coming step is issued at the end of
+  // the step before, so the matrix cores run it during that step's vo
+  // reduction (41.5 vs 43.0 ms per sweep at 64^3; FP64 and FP32 ran slower
+  // that way, 92.8 vs 88.2 and 53.1 vs 53.0 ms)
+  constexpr bool kEarlyM2a = PR::kPacked;
+  acc4 vs0[kTY];
+  if constexpr (kEarlyM2a) m2a(0, PR::kPingPong ? vbA : vbB, vs0);
 
   // One step: this step's operands are vin and xsc, the next step's are
   // fetched into vnx and xsn first thing.
@@ -526,26 +547,18 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
 #pragma unroll
     for (int b = 0; b < kTY; ++b) any_ent[b] = __ballot(ent[b]) != 0;
 
-    // m2a of angle block a: VS[angle][cell] = AT . VI
-    auto m2a = [&](int a, acc4 (&vs)[kTY]) {
-#pragma unroll
-      for (int b = kTY - 1; b >= 0; --b) {
-        vs[b] = acc4{0, 0, 0, 0};
-        if constexpr (PR::kPacked) {
-          vs[b] = PR::mma16b(aop[a], vin[b], vs[b]);
-        } else {
-#pragma unroll
-          for (int s = 0; s < KM; ++s) vs[b] = PR::mma(aop[a][s], vin[b][s], vs[b]);
-        }
-      }
-    };
     // The m2a of angle block a+1 is issued before the solve of block a, so the
     // matrix cores work while the vector units solve.
     acc4 vsc[kTY], vsn[kTY];
-    m2a(0, vsc);
+    if constexpr (kEarlyM2a) {
+#pragma unroll
+      for (int b = 0; b < kTY; ++b) vsc[b] = vs0[b];
+    } else {
+      m2a(0, vin, vsc);
+    }
 #pragma unroll
     for (int a = 0; a < NAB; ++a) {
-      if (a + 1 < NAB) m2a(a + 1 < NAB ? a + 1 : 0, vsn);
+      if (a + 1 < NAB) m2a(a + 1 < NAB ? a + 1 : 0, vin, vsn);
       // solve, blocks in descending order: block b's y face is block b-1's
       // value from the previous step, read before it is overwritten
       // y inflow of row 0 of a lower row group: the row group above wrote it
@@ -608,6 +621,9 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
         for (int b = 0; b < kTY; ++b) vsc[b] = vsn[b];
       }
     }
+    // the next step's block 0 (its operand was fetched at this step's start;
+    // past the last step it runs on clamped data and is never used)
+    if constexpr (kEarlyM2a) m2a(0, vnx, vs0);
     if (wyi == 0) {
       bump(pyi, oyi);
       fetch_yin();
