@@ -838,8 +838,9 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
   if (e == hipSuccess && p->engine == MSW_ENGINE_MFMA && precision != MSW_PREC_FP64) {
     e = hipMalloc(&p->vo32, sizeof(float) * msw_size_state(d));
     p->bytes += sizeof(float) * msw_size_state(d);
-    // the BF16 form reads an FP32 copy of vi (measured faster at 64^3); the
-    // FP32 form reads vi itself, the copy would cost more than it saves
+    // the BF16 form reads a BF16 copy of vi (2 bytes a value, measured faster
+    // at 64^3); the FP32 form reads vi itself, an FP32 copy would cost more
+    // than it saves
     if (e == hipSuccess && precision == MSW_PREC_MIXED_BF16) {
       e = hipMalloc(&p->vib, sizeof(unsigned short) * msw_size_state(d));
       p->bytes += sizeof(unsigned short) * msw_size_state(d);
