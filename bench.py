@@ -188,8 +188,8 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not "
-              "supported (the sweep would shard by energy group across GPUs, "
-              "see DESIGN.md)", file=sys.stderr)
+              "supported (the sweep would shard by energy group across GPUs; "
+              "only the one-GPU share is built)", file=sys.stderr)
         sys.exit(2)
 
     # the reference's CPU run goes first, before this process touches the GPU
