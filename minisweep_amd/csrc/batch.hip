@@ -936,6 +936,9 @@ void finish(MswServer* s, Active& a, int status) {
   r.status = status;
   if (a.req.vi_pinned) (void)hipHostFree(a.req.vi_pinned);
   if (s->opt.keep_outputs && status == MSW_OK) {
+    // a tag kept and never fetched is replaced by the newer result
+    const auto old = s->kept.find(a.req.tag);
+    if (old != s->kept.end()) s->arena.release(old->second.off, old->second.bytes);
     s->kept[a.req.tag] = Kept{a.vo, a.off, a.bytes, a.g.state_doubles()};
   } else if (a.bytes) {
     s->arena.release(a.off, a.bytes);

@@ -265,3 +265,32 @@ def test_batch_more_problems_than_one_launch_holds(cuda):
     d, vi0, vi, vo, a, m = ins[7]
     np.testing.assert_array_equal(vo.cpu().numpy(), _single(torch, d, vi, a, m, msw.PREC_FP64))
     b.close()
+
+
+def test_server_kept_tag_replaced(cuda):
+    """A kept output replaced by a newer result of the same tag gives its
+    arena space back: with room for two problems, tags 7, 7, 8 in turn all
+    run."""
+    print("case server kept tag replaced", flush=True)
+    s = (9, 7, 5, 2, 16, 32, 2)
+    d = msw.dims(*s)
+    probe = msw.Server(precision=msw.PREC_FP64, hbm_budget_bytes=1 << 30, max_active=4,
+                       keep_outputs=True)
+    probe.submit(d, 0)
+    probe.drain(60)
+    probe.poll()
+    one = probe.stats()["arena_peak_bytes"]
+    probe.close()
+    assert one > 0
+    srv = msw.Server(precision=msw.PREC_FP64, hbm_budget_bytes=int(2.5 * one), max_active=4,
+                     keep_outputs=True)
+    for tag in (7, 7, 8):
+        srv.submit(d, tag)
+        srv.drain(30)
+    res = srv.poll()
+    expect = float(closed_form_normsq(*s[:5], s[6]))
+    assert [r["tag"] for r in res] == [7, 7, 8]
+    assert all(r["status"] == 0 and r["normsq"] == expect and r["normsqdiff"] == 0.0
+               for r in res)
+    np.testing.assert_array_equal(srv.fetch(7, d), srv.fetch(8, d))
+    srv.close()
