@@ -129,7 +129,9 @@ int msw_plan_last_timings(MswPlan* plan, float* ms, int n);
 int msw_plan_destroy(MswPlan* plan);
 
 /* normsq = sum vo^2, normsqdiff = sum (vi-vo)^2 in FP64 on the device,
- * written to out2[0], out2[1] (device pointer).  Deterministic order.
+ * written to out2[0], out2[1] (device pointer).  Deterministic order.  Uses a
+ * 16 KB device scratch per calling thread: one thread's calls on different
+ * streams must not overlap.
  * Replaces get_state_norms (src/2_sweeper_base/array_operations.c:69-110). */
 int msw_state_norms(const MswDims* d, const double* vi, const double* vo,
                     double* out2, void* stream);

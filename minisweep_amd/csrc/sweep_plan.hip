@@ -1095,6 +1095,9 @@ int msw_state_norms(const MswDims* d, const double* vi, const double* vo,
   if (rc) return rc;
   if (!vi || !vo || !out2) return set_err(MSW_ERR_INVALID, "null norms argument%s%ld");
   hipStream_t st = (hipStream_t)stream;
+  // block partials: 16 KB per calling thread, allocated on its first call and
+  // kept until the process ends; calls of one thread on different streams
+  // share it, so they must not overlap
   static thread_local double* part = nullptr;
   if (!part) HIP_TRY(hipMalloc(&part, sizeof(double) * 2 * kNormBlocks));
   hipLaunchKernelGGL(k_norms_partial, dim3(kNormBlocks), dim3(kNormThreads), 0, st,
