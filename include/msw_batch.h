@@ -86,7 +86,9 @@ typedef struct {
   int precision;            /* MSW_PREC_* of every request */
   size_t hbm_budget_bytes;  /* device arena for admitted problems */
   int max_active;           /* problems in flight at once (1 = sequential) */
-  int keep_outputs;         /* 1: keep vo in HBM until msw_server_fetch */
+  int keep_outputs;         /* 1: keep vo in HBM until msw_server_fetch (a tag
+                               completed again before its fetch keeps only the
+                               newer result) */
 } MswServerOptions;
 
 typedef struct {
