@@ -680,6 +680,21 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
 #pragma unroll
     for (int b = 0; b < kTY; ++b) {
       if (b % NWA != wa) continue;
+      // Mixed forms with one angle chunk, before the last visit: the FP32
+      // running sum takes one FP32 add, which rounds as the sum in FP64
+      // rounded to FP32 does (FP64 holds more than twice FP32's bits).
+      if constexpr (!kF64 && NWA == 1) {
+        if (!last) {
+          if (val[b] && 4 * g < NM) {
+            float f[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              f[r] = first ? (float)vacc[b][r] : (float)vacc[b][r] + (float)von[b][r];
+            store4(p.vo32 + sc[b] + 4 * g, f);
+          }
+          continue;
+        }
+      }
       double sum[4];
 #pragma unroll
       for (int w = 0; w < NWA; ++w) {
