@@ -231,8 +231,6 @@ int enqueue_tick(int precision, std::vector<TickWork>& work, const SweepParams* 
   return nl;
 }
 
-// Boundary fill of the face planes of several problems (before their first
-// tile launch).  Returns the number of launches.
 // One tick with each kernel variant's launches on a stream of its own
 // (round-robin over ncs streams), forked from st and joined back to it.
 int enqueue_tick_forked(int precision, std::vector<TickWork>& work, const SweepParams* d_params,
@@ -253,6 +251,8 @@ int enqueue_tick_forked(int precision, std::vector<TickWork>& work, const SweepP
   return nl;
 }
 
+// Boundary fill of the face planes of several problems (before their first
+// tile launch).  Returns the number of launches.
 int enqueue_fill(int precision, const std::vector<std::pair<int, const SweepParams*>>& ps,
                  const SweepParams* d_params, hipStream_t st) {
   int nl = 0;
