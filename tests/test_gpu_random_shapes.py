@@ -1,8 +1,9 @@
 """GPU parity over seeded random shapes and plan options, through the C-ABI.
 
 FP64: a perturbed input (vi * (1 + U(-1/4, 1/4))) swept by a plan with
-random angles per wave, tile rows and energy groups per workgroup must match
-the CPU oracle within the FP64 parity bar.  Mixed FP32 / BF16: on the
+random angles per wave, tile rows and energy groups per workgroup, and by a
+z-blocked plan where ncell_z has a proper divisor, must match the CPU oracle
+within the FP64 parity bar.  Mixed FP32 / BF16: on the
 manufactured input every operand is exact in both forms, so vo == vi and
 normsq equals its closed form exactly, for any shape.
 
@@ -65,7 +66,12 @@ def test_fp64_random_shape_vs_oracle(shape, cuda):
     ta, tm = torch.from_numpy(a).to(cuda), torch.from_numpy(m).to(cuda)
     vi = torch.from_numpy(vi0).to(cuda)
     nac, rows, epw = _options(shape, rng)
-    for opts in ({}, dict(angles_per_wave=nac, tile_rows=rows, groups_per_workgroup=epw)):
+    runs = [{}, dict(angles_per_wave=nac, tile_rows=rows, groups_per_workgroup=epw)]
+    # z blocks: the largest proper divisor of ncell_z, if any
+    zb = max([k for k in range(1, shape[2]) if shape[2] % k == 0], default=0)
+    if zb:
+        runs.append(dict(angles_per_wave=nac, tile_rows=rows, z_block=zb))
+    for opts in runs:
         vo = torch.full_like(vi, float("nan"))
         plan = msw.Plan(d, msw.PREC_FP64, engine=msw.ENGINE_MFMA, **opts)
         plan.sweep(vi, vo, ta, tm)
