@@ -53,10 +53,6 @@ int set_error(int code, const std::string& msg) {
   return code;
 }
 
-// Default angles per wave of the MFMA kernels: measured at 64^3, na=64, 32
-// per wave is fastest in FP64 (16 loses more to the cross-wave vo reduction
-// than it gains in occupancy) and 64 in the mixed forms; never more than na
-// rounded up to a 16-angle block, and at most 4 waves per tile.
 // Energy groups streamed through one MFMA tile pipeline: about 128 stream
 // steps per workgroup, so the 18-step fill and drain of a tile cost ~12%
 // rather than 18/(nz+18).
@@ -88,6 +84,10 @@ int launch_mfma_epw(int preferred, int ne, long long waves1, int precision) {
   return e;
 }
 
+// Default angles per wave of the MFMA kernels: measured at 64^3, na=64, 32
+// per wave is fastest in FP64 (16 loses more to the cross-wave vo reduction
+// than it gains in occupancy) and 64 in the mixed forms; never more than na
+// rounded up to a 16-angle block, and at most 4 waves per tile.
 int default_mfma_nac(int precision, int na) {
   const int cap = na <= 16 ? 16 : na <= 32 ? 32 : 64;
   int nac = precision == MSW_PREC_FP64 ? 32 : 64;
@@ -592,12 +592,14 @@ struct MswPlan {
   // in the first; fz the z faces between blocks
   int zb = 0, nblk = 1, pass_split = 0;
   void* fz = nullptr;
-  // host-streamed sweeps: device slabs (double-buffered), copy stream, events
+  // host-streamed sweeps: device slabs (double-buffered), a stream for copies
+  // in and one for copies out (PCIe carries both directions at once), events
   double* slab_vi[2] = {nullptr, nullptr};
   double* slab_vo[2] = {nullptr, nullptr};
   int nslot = 0;
-  hipStream_t cst = nullptr;
+  hipStream_t cst = nullptr, cst_out = nullptr;
   hipEvent_t ev_in[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr};
+  hipEvent_t ev_out[2] = {nullptr, nullptr};
 };
 
 template <int NM, int NAC>
@@ -913,13 +915,16 @@ int msw_plan_destroy(MswPlan* p) {
   if (p->items) (void)hipFree(p->items);
   if (p->fz) (void)hipFree(p->fz);
   if (p->cst) (void)hipStreamSynchronize(p->cst);
+  if (p->cst_out) (void)hipStreamSynchronize(p->cst_out);
   for (int k = 0; k < 2; ++k) {
     if (p->slab_vi[k]) (void)hipFree(p->slab_vi[k]);
     if (p->slab_vo[k]) (void)hipFree(p->slab_vo[k]);
     if (p->ev_in[k]) (void)hipEventDestroy(p->ev_in[k]);
     if (p->ev_done[k]) (void)hipEventDestroy(p->ev_done[k]);
+    if (p->ev_out[k]) (void)hipEventDestroy(p->ev_out[k]);
   }
   if (p->cst) (void)hipStreamDestroy(p->cst);
+  if (p->cst_out) (void)hipStreamDestroy(p->cst_out);
   delete p;
   return MSW_OK;
 }
@@ -996,16 +1001,18 @@ int msw_plan_sweep_host(MswPlan* p, const double* vi_host, double* vo_host,
   hipStream_t st = (hipStream_t)stream;
   const MswDims& d = p->d;
   const size_t slab = msw_size_state(&d) / p->nblk;
-  if (!p->nslot) {  // first call: slabs, copy stream, events
+  if (!p->nslot) {  // first call: slabs, copy streams, events
     p->nslot = p->nblk > 1 ? 2 : 1;
     for (int k = 0; k < p->nslot; ++k) {
       HIP_TRY(hipMalloc(&p->slab_vi[k], sizeof(double) * slab));
       HIP_TRY(hipMalloc(&p->slab_vo[k], sizeof(double) * slab));
       HIP_TRY(hipEventCreateWithFlags(&p->ev_in[k], hipEventDisableTiming));
       HIP_TRY(hipEventCreateWithFlags(&p->ev_done[k], hipEventDisableTiming));
+      HIP_TRY(hipEventCreateWithFlags(&p->ev_out[k], hipEventDisableTiming));
       p->bytes += 2 * sizeof(double) * slab;
     }
     HIP_TRY(hipStreamCreateWithFlags(&p->cst, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&p->cst_out, hipStreamNonBlocking));
   }
   const int n = kNoct * d.nm * d.na;
   hipLaunchKernelGGL(k_pack_matrices, dim3((n + 255) / 256), dim3(256), 0, st,
@@ -1021,10 +1028,17 @@ int msw_plan_sweep_host(MswPlan* p, const double* vi_host, double* vo_host,
         steps.push_back(Step{pass, kk, pass == 0 ? kk : p->nblk - 1 - kk});
   const size_t bytes = sizeof(double) * slab;
   // host -> device for step i into its slot: vi, and vo summed so far in the
-  // second pass
+  // second pass.  It waits for the copy out of the step that last used the
+  // slot (i - nslot; that copy followed the step's compute), and at the turn
+  // between passes for the copy out of step i - 1, the same block.  The copies
+  // out run in step order on their own stream, so a block's first-pass vo is
+  // back on the host before its second pass reads it.
   auto copy_in = [&](size_t i) -> int {
     const Step& s = steps[i];
     const int sl = (int)(i % p->nslot);
+    if (i >= (size_t)p->nslot) HIP_TRY(hipStreamWaitEvent(p->cst, p->ev_out[sl], 0));
+    if (i >= 1 && steps[i - 1].k == s.k)
+      HIP_TRY(hipStreamWaitEvent(p->cst, p->ev_out[(i - 1) % p->nslot], 0));
     HIP_TRY(hipMemcpyAsync(p->slab_vi[sl], vi_host + (size_t)s.k * slab, bytes,
                            hipMemcpyHostToDevice, p->cst));
     if (s.pass == 1)
@@ -1033,9 +1047,9 @@ int msw_plan_sweep_host(MswPlan* p, const double* vi_host, double* vo_host,
     HIP_TRY(hipEventRecord(p->ev_in[sl], p->cst));
     return MSW_OK;
   };
-  // Copies run on the plan's copy stream: the next step's inputs go in while
-  // this step computes, except where the next step is the same block (the
-  // turn between passes), whose vo must come back first.
+  // The next step's inputs go in while this step computes, except where the
+  // next step is the same block (the turn between passes), whose vo must come
+  // back first.
   int rc = copy_in(0);
   if (rc) return rc;
   for (size_t i = 0; i < steps.size(); ++i) {
@@ -1078,14 +1092,14 @@ int msw_plan_sweep_host(MswPlan* p, const double* vi_host, double* vo_host,
     }
     if (rc) return rc;
     HIP_TRY(hipEventRecord(p->ev_done[sl], st));
-    HIP_TRY(hipStreamWaitEvent(p->cst, p->ev_done[sl], 0));
+    HIP_TRY(hipStreamWaitEvent(p->cst_out, p->ev_done[sl], 0));
     HIP_TRY(hipMemcpyAsync(vo_host + (size_t)s.k * slab, p->slab_vo[sl], bytes,
-                           hipMemcpyDeviceToHost, p->cst));
+                           hipMemcpyDeviceToHost, p->cst_out));
+    HIP_TRY(hipEventRecord(p->ev_out[sl], p->cst_out));
     if (turn && (rc = copy_in(i + 1))) return rc;
   }
-  // the caller's stream covers the last copy back
-  HIP_TRY(hipEventRecord(p->ev_in[0], p->cst));
-  HIP_TRY(hipStreamWaitEvent(st, p->ev_in[0], 0));
+  // the caller's stream covers the last copy back (and so every copy out)
+  HIP_TRY(hipStreamWaitEvent(st, p->ev_out[(steps.size() - 1) % p->nslot], 0));
   return MSW_OK;
 }
 
