@@ -109,9 +109,11 @@ int msw_plan_sweep(MswPlan* plan, const double* vi, double* vo,
  * in the state layout, pinned (hipHostMalloc) for the copies to overlap the
  * sweep.  The state goes through HBM one z block at a time, each block's vi
  * (and, in the second pass, its partial vo) copied in while the previous
- * block is swept, its vo copied back after: two passes of four octants, 5
- * state volumes over PCIe per sweep (vi twice, vo back twice, vo in once).  Device memory: two slabs of vi and vo
- * of one block each, plus the plan.  a_from_m, m_from_a are device arrays.
+ * block is swept, its vo copied back after: two passes of four octants.
+ * With n z blocks, 5 - 3/n state volumes cross PCIe per sweep (vi twice, vo
+ * back twice, vo in once, less the block at the turn between the passes,
+ * which stays on the device); copies in and copies out overlap.  Device
+ * memory: two slabs of vi and vo of one block each, plus the plan.  a_from_m, m_from_a are device arrays.
  * Enqueued on stream; synchronize it before reading vo_host. */
 int msw_plan_sweep_host(MswPlan* plan, const double* vi_host, double* vo_host,
                         const double* a_from_m, const double* m_from_a, void* stream);

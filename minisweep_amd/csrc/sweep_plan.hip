@@ -1027,18 +1027,23 @@ int msw_plan_sweep_host(MswPlan* p, const double* vi_host, double* vo_host,
       for (int kk = 0; kk < p->nblk; ++kk)
         steps.push_back(Step{pass, kk, pass == 0 ? kk : p->nblk - 1 - kk});
   const size_t bytes = sizeof(double) * slab;
+  // Slots alternate, except that the second pass starts in the slot where the
+  // first pass ended: that step (the turn) sweeps the same block again, and
+  // its vi and partial vo are still there, so nothing crosses PCIe for it and
+  // the first pass's vo of that block never goes out.
+  auto slot_of = [&](size_t i) -> int {
+    return p->nslot == 1 ? 0 : (int)((steps[i].pass == 1 ? i - 1 : i) % 2);
+  };
+  bool out_recorded[2] = {false, false};
   // host -> device for step i into its slot: vi, and vo summed so far in the
-  // second pass.  It waits for the copy out of the step that last used the
-  // slot (i - nslot; that copy followed the step's compute), and at the turn
-  // between passes for the copy out of step i - 1, the same block.  The copies
-  // out run in step order on their own stream, so a block's first-pass vo is
-  // back on the host before its second pass reads it.
+  // second pass.  It waits for the last copy out from the slot (which followed
+  // the compute that last used it).  The copies out run in step order on their
+  // own stream, so a block's first-pass vo is back on the host before its
+  // second pass reads it.
   auto copy_in = [&](size_t i) -> int {
     const Step& s = steps[i];
-    const int sl = (int)(i % p->nslot);
-    if (i >= (size_t)p->nslot) HIP_TRY(hipStreamWaitEvent(p->cst, p->ev_out[sl], 0));
-    if (i >= 1 && steps[i - 1].k == s.k)
-      HIP_TRY(hipStreamWaitEvent(p->cst, p->ev_out[(i - 1) % p->nslot], 0));
+    const int sl = slot_of(i);
+    if (out_recorded[sl]) HIP_TRY(hipStreamWaitEvent(p->cst, p->ev_out[sl], 0));
     HIP_TRY(hipMemcpyAsync(p->slab_vi[sl], vi_host + (size_t)s.k * slab, bytes,
                            hipMemcpyHostToDevice, p->cst));
     if (s.pass == 1)
@@ -1047,18 +1052,17 @@ int msw_plan_sweep_host(MswPlan* p, const double* vi_host, double* vo_host,
     HIP_TRY(hipEventRecord(p->ev_in[sl], p->cst));
     return MSW_OK;
   };
-  // The next step's inputs go in while this step computes, except where the
-  // next step is the same block (the turn between passes), whose vo must come
-  // back first.
+  // The next step's inputs go in while this step computes (none for the turn).
   int rc = copy_in(0);
   if (rc) return rc;
   for (size_t i = 0; i < steps.size(); ++i) {
     const Step& s = steps[i];
-    const int sl = (int)(i % p->nslot);
+    const int sl = slot_of(i);
     const bool next = i + 1 < steps.size();
-    const bool turn = next && steps[i + 1].k == s.k;
+    const bool turn = next && steps[i + 1].k == s.k;  // the next step resumes this slot
+    const bool resumed = i >= 1 && steps[i - 1].k == s.k;
     if (next && !turn && (rc = copy_in(i + 1))) return rc;
-    HIP_TRY(hipStreamWaitEvent(st, p->ev_in[sl], 0));
+    if (!resumed) HIP_TRY(hipStreamWaitEvent(st, p->ev_in[sl], 0));
     if (s.pass < 0) {
       rc = sweep_device(p, p->slab_vi[sl], p->slab_vo[sl], st);
     } else {
@@ -1091,15 +1095,17 @@ int msw_plan_sweep_host(MswPlan* p, const double* vi_host, double* vo_host,
                         s.pass == 0 ? p->pass_split : p->launch_off.size());
     }
     if (rc) return rc;
-    HIP_TRY(hipEventRecord(p->ev_done[sl], st));
-    HIP_TRY(hipStreamWaitEvent(p->cst_out, p->ev_done[sl], 0));
-    HIP_TRY(hipMemcpyAsync(vo_host + (size_t)s.k * slab, p->slab_vo[sl], bytes,
-                           hipMemcpyDeviceToHost, p->cst_out));
-    HIP_TRY(hipEventRecord(p->ev_out[sl], p->cst_out));
-    if (turn && (rc = copy_in(i + 1))) return rc;
+    if (!turn) {
+      HIP_TRY(hipEventRecord(p->ev_done[sl], st));
+      HIP_TRY(hipStreamWaitEvent(p->cst_out, p->ev_done[sl], 0));
+      HIP_TRY(hipMemcpyAsync(vo_host + (size_t)s.k * slab, p->slab_vo[sl], bytes,
+                             hipMemcpyDeviceToHost, p->cst_out));
+      HIP_TRY(hipEventRecord(p->ev_out[sl], p->cst_out));
+      out_recorded[sl] = true;
+    }
   }
   // the caller's stream covers the last copy back (and so every copy out)
-  HIP_TRY(hipStreamWaitEvent(st, p->ev_out[(steps.size() - 1) % p->nslot], 0));
+  HIP_TRY(hipStreamWaitEvent(st, p->ev_out[slot_of(steps.size() - 1)], 0));
   return MSW_OK;
 }
 
