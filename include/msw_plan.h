@@ -113,8 +113,10 @@ int msw_plan_sweep(MswPlan* plan, const double* vi, double* vo,
  * With n z blocks, 5 - 3/n state volumes cross PCIe per sweep (vi twice, vo
  * back twice, vo in once, less the block at the turn between the passes,
  * which stays on the device); copies in and copies out overlap.  Device
- * memory: two slabs of vi and vo of one block each, plus the plan.  a_from_m, m_from_a are device arrays.
- * Enqueued on stream; synchronize it before reading vo_host. */
+ * memory: two slabs of vi and vo of one block each, plus the plan.
+ * a_from_m, m_from_a are device arrays.  Enqueued on stream; synchronize it
+ * before reading vo_host.  Calls on one plan must not overlap (they share the
+ * slabs and face planes). */
 int msw_plan_sweep_host(MswPlan* plan, const double* vi_host, double* vo_host,
                         const double* a_from_m, const double* m_from_a, void* stream);
 
