@@ -99,7 +99,8 @@ int msw_plan_num_launches(const MswPlan* plan);
 
 /* One sweep vo <- S(vi), all arrays resident in HBM, enqueued on stream.
  * Replaces Sweeper_sweep (src/3_sweeper/sweeper_kba_c.h:588-845).
- * vo need not be zeroed; vi is not modified. */
+ * vo need not be zeroed; vi is not modified.  Sweeps of one plan on
+ * different streams must not overlap: they share its face planes. */
 int msw_plan_sweep(MswPlan* plan, const double* vi, double* vo,
                    const double* a_from_m, const double* m_from_a,
                    void* stream);
