@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -41,9 +42,13 @@ namespace msw {
 
 static thread_local std::string g_err;
 
-static int set_err(int code, const char* fmt, const char* a = "", long b = 0) {
+static int set_err(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+static int set_err(int code, const char* fmt, ...) {
   char buf[512];
-  snprintf(buf, sizeof(buf), fmt, a, b);
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
   g_err = buf;
   return code;
 }
@@ -666,10 +671,10 @@ double msw_flops(const MswDims* d, int niterations) {
 }
 
 static int check_dims(const MswDims* d) {
-  if (!d) return set_err(MSW_ERR_INVALID, "null dims%s%ld");
+  if (!d) return set_err(MSW_ERR_INVALID, "null dims");
   if (d->ncell_x <= 0 || d->ncell_y <= 0 || d->ncell_z <= 0 || d->ne <= 0 ||
       d->nm <= 0 || d->na <= 0 || d->nu <= 0)
-    return set_err(MSW_ERR_INVALID, "all dimensions must be positive%s%ld");
+    return set_err(MSW_ERR_INVALID, "all dimensions must be positive");
   return MSW_OK;
 }
 
@@ -700,23 +705,22 @@ int msw_plan_create(const MswDims* d, int precision, MswPlan** out) {
 }
 
 int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** out) {
-  if (!out) return set_err(MSW_ERR_INVALID, "null plan pointer%s%ld");
+  if (!out) return set_err(MSW_ERR_INVALID, "null plan pointer");
   *out = nullptr;
   int rc = check_dims(d);
   if (rc) return rc;
-  if (!opt) return set_err(MSW_ERR_INVALID, "null options%s%ld");
+  if (!opt) return set_err(MSW_ERR_INVALID, "null options");
   const int precision = opt->precision;
   if (precision != MSW_PREC_FP64 && precision != MSW_PREC_MIXED_FP32 &&
       precision != MSW_PREC_MIXED_BF16)
-    return set_err(MSW_ERR_INVALID, "unknown precision %s%ld", "", precision);
+    return set_err(MSW_ERR_INVALID, "unknown precision %d", precision);
   if (precision != MSW_PREC_FP64 && opt->engine != MSW_ENGINE_MFMA)
     return set_err(MSW_ERR_UNSUPPORTED,
-                   "mixed precision runs on the MFMA engine only (precision %s%ld)", "",
-                   precision);
+                   "mixed precision runs on the MFMA engine only (precision %d)", precision);
   if (d->nm != 4 && d->nm != 8 && d->nm != 16)
-    return set_err(MSW_ERR_UNSUPPORTED, "nm must be 4, 8 or 16 (got %s%ld)", "", d->nm);
+    return set_err(MSW_ERR_UNSUPPORTED, "nm must be 4, 8 or 16 (got %d)", d->nm);
   if (opt->engine != MSW_ENGINE_VALU && opt->engine != MSW_ENGINE_MFMA)
-    return set_err(MSW_ERR_INVALID, "unknown engine %s%ld", "", opt->engine);
+    return set_err(MSW_ERR_INVALID, "unknown engine %d", opt->engine);
   MswPlan* p = new MswPlan();
   p->d = *d;
   p->precision = precision;
@@ -728,21 +732,21 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
     if (p->nac != 16 && p->nac != 32 && p->nac != 64) {
       delete p;
       return set_err(MSW_ERR_INVALID,
-                     "MFMA engine: angles_per_wave must be 16, 32 or 64 (got %s%ld)", "", p->nac);
+                     "MFMA engine: angles_per_wave must be 16, 32 or 64 (got %d)", p->nac);
     }
   } else {
     p->nac = opt->angles_per_wave ? opt->angles_per_wave : pick_nac(d->na);
     if (p->nac != 8 && p->nac != 16 && p->nac != 32) {
       delete p;
-      return set_err(MSW_ERR_INVALID, "VALU engine: angles_per_wave must be 8, 16 or 32 (got %s%ld)",
-                     "", p->nac);
+      return set_err(MSW_ERR_INVALID, "VALU engine: angles_per_wave must be 8, 16 or 32 (got %d)",
+                     p->nac);
     }
   }
   p->nwa = (d->na + p->nac - 1) / p->nac;
   if (p->nwa > 4) {
     delete p;
-    return set_err(MSW_ERR_UNSUPPORTED, "na above 4 waves of angles unsupported (na=%s%ld)",
-                   "", d->na);
+    return set_err(MSW_ERR_UNSUPPORTED, "na above 4 waves of angles unsupported (na=%d)",
+                   d->na);
   }
   p->na_pad = p->nwa * p->nac;
   if (opt->groups_per_workgroup < 0) {
@@ -780,9 +784,9 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
     if (opt->tile_rows % mfma_tile_rows() != 0 || !mfma_shape_supported(p->nwa, p->ny)) {
       delete p;
       return set_err(MSW_ERR_UNSUPPORTED,
-                     "tile_rows %s%ld is not built for this angle chunking (4, 8 or 16 with at "
+                     "tile_rows %d is not built for this angle chunking (4, 8 or 16 with at "
                      "most 4 waves per workgroup)",
-                     "", opt->tile_rows);
+                     opt->tile_rows);
     }
   }
   const int ty = p->engine == MSW_ENGINE_MFMA ? mfma_tile_rows() * p->ny : kTile;
@@ -856,7 +860,7 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
   if (e != hipSuccess) {
     msw_plan_destroy(p);
     return set_err(e == hipErrorOutOfMemory ? MSW_ERR_NOMEM : MSW_ERR_HIP,
-                   "plan allocation failed: %s%ld", hipGetErrorString(e));
+                   "plan allocation failed: %s", hipGetErrorString(e));
   }
   *out = p;
   return MSW_OK;
@@ -869,7 +873,7 @@ int msw_plan_num_launches(const MswPlan* p) {
 }
 
 int msw_plan_set_timing(MswPlan* p, int enable) {
-  if (!p) return set_err(MSW_ERR_INVALID, "null plan%s%ld");
+  if (!p) return set_err(MSW_ERR_INVALID, "null plan");
   if (enable && p->ev_start.empty()) {
     const size_t n = p->launch_off.size();
     p->ev_start.resize(n);
@@ -884,9 +888,9 @@ int msw_plan_set_timing(MswPlan* p, int enable) {
 }
 
 static int last_timings(MswPlan* p, float* ms, int n, int* count) {
-  if (!p || !ms) return set_err(MSW_ERR_INVALID, "null argument%s%ld");
+  if (!p || !ms) return set_err(MSW_ERR_INVALID, "null argument");
   if (!p->timing || p->ev_stop.empty())
-    return set_err(MSW_ERR_INVALID, "timing is not enabled on this plan%s%ld");
+    return set_err(MSW_ERR_INVALID, "timing is not enabled on this plan");
   const int m = std::min<int>(n, (int)p->ev_stop.size());
   HIP_TRY(hipEventSynchronize(p->ev_stop.back()));
   for (int l = 0; l < m; ++l)
@@ -983,8 +987,8 @@ static int sweep_device(MswPlan* p, const double* vi, double* vo, hipStream_t st
 int msw_plan_sweep(MswPlan* p, const double* vi, double* vo,
                    const double* a_from_m, const double* m_from_a, void* stream) {
   if (!p || !vi || !vo || !a_from_m || !m_from_a)
-    return set_err(MSW_ERR_INVALID, "null argument to msw_plan_sweep%s%ld");
-  if (vi == vo) return set_err(MSW_ERR_INVALID, "vi and vo must differ%s%ld");
+    return set_err(MSW_ERR_INVALID, "null argument to msw_plan_sweep");
+  if (vi == vo) return set_err(MSW_ERR_INVALID, "vi and vo must differ");
   hipStream_t st = (hipStream_t)stream;
   const MswDims& d = p->d;
   const int n = kNoct * d.nm * d.na;
@@ -1113,7 +1117,7 @@ int msw_state_norms(const MswDims* d, const double* vi, const double* vo,
                     double* out2, void* stream) {
   int rc = check_dims(d);
   if (rc) return rc;
-  if (!vi || !vo || !out2) return set_err(MSW_ERR_INVALID, "null norms argument%s%ld");
+  if (!vi || !vo || !out2) return set_err(MSW_ERR_INVALID, "null norms argument");
   hipStream_t st = (hipStream_t)stream;
   // block partials: 16 KB per calling thread, allocated on its first call and
   // kept until the process ends; calls of one thread on different streams
@@ -1130,7 +1134,7 @@ int msw_state_norms(const MswDims* d, const double* vi, const double* vo,
 int msw_init_state(const MswDims* d, double* v, void* stream) {
   int rc = check_dims(d);
   if (rc) return rc;
-  if (!v) return set_err(MSW_ERR_INVALID, "null state%s%ld");
+  if (!v) return set_err(MSW_ERR_INVALID, "null state");
   const size_t n = msw_size_state(d);
   const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
   hipLaunchKernelGGL(k_init_state, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
@@ -1141,7 +1145,7 @@ int msw_init_state(const MswDims* d, double* v, void* stream) {
 
 int msw_init_matrices_host(int nm, int na, double* A, double* M) {
   if (nm <= 0 || na <= 0 || !A || !M)
-    return set_err(MSW_ERR_INVALID, "bad matrix init arguments%s%ld");
+    return set_err(MSW_ERR_INVALID, "bad matrix init arguments");
   // Quantities_init_am_matrices_ (src/3_sweeper/quantities_testing_c.h:40-201):
   // integer rows that send an affine vector in moments to an affine vector in
   // angles and back, plus rows that annihilate affine vectors.

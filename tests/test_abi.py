@@ -85,10 +85,14 @@ def test_flops_and_sizes():
 
 
 def test_plan_rejects_bad_dims_without_gpu():
-    with pytest.raises(msw.SweepError, match="positive"):
+    """Argument errors are caught before any device work, with the message
+    text exact (nothing appended)."""
+    with pytest.raises(msw.SweepError, match=r"all dimensions must be positive$"):
         msw.Plan(msw.dims(0, 4, 4, 1, 16, 8, 4))
-    with pytest.raises(msw.SweepError, match="nm must be"):
+    with pytest.raises(msw.SweepError, match=r"nm must be 4, 8 or 16 \(got 5\)$"):
         msw.Plan(msw.dims(4, 4, 4, 1, 5, 8, 4))
+    with pytest.raises(msw.SweepError, match=r"unknown precision 7$"):
+        msw.Plan(msw.dims(4, 4, 4, 1, 16, 8, 4), precision=7)
 
 
 def _sweep_cli(*args):
