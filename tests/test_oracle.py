@@ -59,8 +59,8 @@ def test_oracle_manufactured_exact(shape):
     assert n == float(closed_form_normsq(ncx, ncy, ncz, ne, nm, 4))
 
 
-def test_closed_form_matches_baseline_numbers():
-    # BASELINE known answers: 8^3 ne4 nm16 and 64^3 ne64 nm16
+def test_closed_form_known_answers():
+    # the manufactured problem's normsq: 8^3 ne4 nm16 and 64^3 ne64 nm16
     assert closed_form_normsq(8, 8, 8, 4, 16) == 188156108800
     assert closed_form_normsq(64, 64, 64, 64, 16) == 1541374843289600
     # sweep with no arguments, scripts' NM=4 (reference printed 6.89525100e+09)
