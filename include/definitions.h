@@ -1,0 +1,17 @@
+/*
+ * definitions.h -- reference header name for callers written against the
+ * reference (it replaces src/2_sweeper_base/definitions.h).  The declarations it stands for,
+ * NDIM, NOCTANT, Dir_* (declared in sweeper.h), live in sweeper.h; this file only forwards there, so that the
+ * reference's unchanged drivers (src/4_driver/sweep.c:13-22, runner.c:13-22,
+ * tester.c:13-22) compile against this library's include directory.
+ */
+#ifndef MSW_FWD_DEFINITIONS_H
+#define MSW_FWD_DEFINITIONS_H
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "sweeper.h"
+
+#endif

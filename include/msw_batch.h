@@ -37,7 +37,8 @@ extern "C" {
 typedef struct MswBatch MswBatch;
 
 typedef struct {
-  int precision;     /* MSW_PREC_* for every problem of the batch */
+  int precision;     /* MSW_PREC_* for every problem of the batch (default FP64;
+                        MSW_PREC_MIXED_FP32 and _BF16 are lossy, see msw_plan.h) */
   int use_graph;     /* 1 (default): run = one replay of a captured hipGraph */
   int schedule_kind; /* tile schedule heuristic: 0 most-remaining octant first, 1 the
                         reference's octant order */
@@ -83,12 +84,15 @@ int msw_batch_destroy(MswBatch* batch);
 typedef struct MswServer MswServer;
 
 typedef struct {
-  int precision;            /* MSW_PREC_* of every request */
+  int precision;            /* MSW_PREC_* of every request (default FP64) */
   size_t hbm_budget_bytes;  /* device arena for admitted problems */
   int max_active;           /* problems in flight at once (1 = sequential) */
   int keep_outputs;         /* 1: keep vo in HBM until msw_server_fetch (a tag
                                completed again before its fetch keeps only the
                                newer result) */
+  int schedule_kind;        /* tile schedule heuristic, as MswPlanOptions (0 most
+                               remaining octant first, 1 the reference's octant
+                               order); msw_server_tune_from_batch replaces it */
 } MswServerOptions;
 
 typedef struct {

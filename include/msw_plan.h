@@ -79,6 +79,10 @@ typedef struct {
                                (must divide ncell_z; 0: all planes at once).  The
                                reference's --nblock_z; needed by msw_plan_sweep_host
                                for states larger than HBM */
+  int schedule_kind;        /* order of the (octant, tile) visits: 0 (default) most
+                               remaining visits first; 1 the reference's KBA octant
+                               order {0,4,2,6,3,7,1,5}
+                               (src/3_sweeper/stepscheduler_kba.c:155) */
 } MswPlanOptions;
 
 void msw_plan_default_options(MswPlanOptions* opt);

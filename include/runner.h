@@ -6,6 +6,7 @@
 #define MSW_RUNNER_H
 
 #include "sweeper.h"
+#include "array_operations.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -33,13 +34,6 @@ void Runner_run_case(Runner* runner, Arguments* args, Env* env);
 /* compare_runs (src/4_driver/runner.c:187-241): two runs must both give
  * normsqdiff == 0 and equal normsq, compared exactly. */
 Bool_t compare_runs(const char* argstring1, const char* argstring2, Env* env);
-
-/* src/2_sweeper_base/array_operations.c:26-126 (host arrays) */
-void initialize_state(P* v, const Dimensions dims, int nu, const Quantities* quan);
-void initialize_state_zero(P* v, const Dimensions dims, int nu);
-void get_state_norms(const P* vi, const P* vo, const Dimensions dims, int nu,
-                     P* normsqp, P* normsqdiffp, Env* env);
-void copy_vector(P* vo, const P* vi, size_t n);
 
 #ifdef __cplusplus
 }

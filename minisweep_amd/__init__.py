@@ -39,7 +39,8 @@ class MswDims(ctypes.Structure):
 class MswPlanOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("engine", ctypes.c_int),
                 ("angles_per_wave", ctypes.c_int), ("groups_per_workgroup", ctypes.c_int),
-                ("tile_rows", ctypes.c_int), ("z_block", ctypes.c_int)]
+                ("tile_rows", ctypes.c_int), ("z_block", ctypes.c_int),
+                ("schedule_kind", ctypes.c_int)]
 
 
 class MswBatchOptions(ctypes.Structure):
@@ -49,7 +50,8 @@ class MswBatchOptions(ctypes.Structure):
 
 class MswServerOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("hbm_budget_bytes", ctypes.c_size_t),
-                ("max_active", ctypes.c_int), ("keep_outputs", ctypes.c_int)]
+                ("max_active", ctypes.c_int), ("keep_outputs", ctypes.c_int),
+                ("schedule_kind", ctypes.c_int)]
 
 
 class MswResult(ctypes.Structure):
@@ -175,7 +177,8 @@ class Plan:
 
     def __init__(self, d: MswDims, precision: int = PREC_FP64,
                  engine: Optional[int] = None, angles_per_wave: int = 0,
-                 groups_per_workgroup: int = 0, tile_rows: int = 0, z_block: int = 0):
+                 groups_per_workgroup: int = 0, tile_rows: int = 0, z_block: int = 0,
+                 schedule_kind: int = 0):
         self.d = d
         self.precision = precision
         o = MswPlanOptions()
@@ -187,6 +190,7 @@ class Plan:
         o.groups_per_workgroup = groups_per_workgroup
         o.tile_rows = tile_rows
         o.z_block = z_block
+        o.schedule_kind = schedule_kind
         self.engine = o.engine
         h = ctypes.c_void_p()
         _check(lib().msw_plan_create_ex(ctypes.byref(d), ctypes.byref(o), ctypes.byref(h)),
@@ -264,7 +268,7 @@ class Batch:
     """Many independent sweeps run together (msw_batch.h): one hipGraph replay
     per run, kernel variants chosen per (nm, na) class, optionally autotuned."""
 
-    def __init__(self, dims_list, precision: int = PREC_MIXED_FP32, use_graph: bool = True,
+    def __init__(self, dims_list, precision: int = PREC_FP64, use_graph: bool = True,
                  schedule_kind: int = 0, stagger: bool = True):
         o = MswBatchOptions()
         lib().msw_batch_default_options(ctypes.byref(o))
@@ -321,12 +325,13 @@ class Server:
     """Request loop with continuous batching (msw_batch.h): submit problems,
     poll completions (FP64 norms and server-clock timestamps)."""
 
-    def __init__(self, precision: int = PREC_MIXED_FP32, hbm_budget_bytes: int = 32 << 30,
-                 max_active: int = 512, keep_outputs: bool = False):
+    def __init__(self, precision: int = PREC_FP64, hbm_budget_bytes: int = 32 << 30,
+                 max_active: int = 512, keep_outputs: bool = False, schedule_kind: int = 0):
         o = MswServerOptions()
         lib().msw_server_default_options(ctypes.byref(o))
         o.precision, o.hbm_budget_bytes = precision, hbm_budget_bytes
         o.max_active, o.keep_outputs = max_active, int(keep_outputs)
+        o.schedule_kind = schedule_kind
         h = ctypes.c_void_p()
         _check(lib().msw_server_create(ctypes.byref(o), ctypes.byref(h)), "msw_server_create")
         self._h = h

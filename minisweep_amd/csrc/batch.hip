@@ -200,8 +200,23 @@ void choose_tick_epw(std::vector<TickWork>& work, int precision) {
   }
 }
 
+// Host-side checks of one problem's share of a tick before it goes into a
+// MultiLaunch: the launch exists in its schedule, the stream length is
+// positive, and its workgroups fit the kernel's int indexing.  Returns the
+// workgroup count, or -1 (and sets the error text).
+long long tick_workgroups(const TickWork& w) {
+  if (!w.g || !w.s || w.launch < 0 || w.launch >= w.s->nlaunch() || w.epw < 1 ||
+      w.s->cnt[w.launch] < 1 || w.s->off[w.launch] < 0) {
+    set_error(MSW_ERR_INVALID, "internal: tick work outside its schedule");
+    return -1;
+  }
+  return (long long)w.s->cnt[w.launch] * ((w.g->d.ne + w.epw - 1) / w.epw) * w.g->d.nu;
+}
+
 // Enqueue one combined tick: per kernel variant, k_sweep_mfma_multi launches
-// of up to kMaxSegments problems each.  Returns the number of launches.
+// of up to kMaxSegments problems each.  Returns the number of launches, or a
+// negated MSW_ERR_* code when the assembly checks fail (nothing of the
+// offending launch is enqueued).
 int enqueue_tick(int precision, std::vector<TickWork>& work, const SweepParams* d_params,
                  hipStream_t st) {
   choose_tick_epw(work, precision);
@@ -216,12 +231,23 @@ int enqueue_tick(int precision, std::vector<TickWork>& work, const SweepParams* 
       long long grid = 0;
       for (size_t k = b; k < ws.size() && k < b + kMaxSegments; ++k) {
         const TickWork& w = *ws[k];
+        const long long nwg = tick_workgroups(w);
+        if (nwg < 0) return -MSW_ERR_INVALID;
+        if (ml.nseg >= kMaxSegments || w.slot < 0) {
+          set_error(MSW_ERR_INVALID, "internal: segment table overflow");
+          return -MSW_ERR_INVALID;
+        }
         ml.start[ml.nseg] = (int)grid;
         ml.prob[ml.nseg] = w.slot;
         ml.item_base[ml.nseg] = w.s->off[w.launch];
         ml.epw[ml.nseg] = w.epw;
         ml.nseg++;
-        grid += (long long)w.s->cnt[w.launch] * ((w.g->d.ne + w.epw - 1) / w.epw) * w.g->d.nu;
+        grid += nwg;
+      }
+      if (grid < 1 || grid > 0x7fffffffll) {
+        set_error(MSW_ERR_INVALID, "internal: combined launch of " + std::to_string(grid) +
+                                       " workgroups");
+        return -MSW_ERR_INVALID;
       }
       const Geom& g = *ws[b]->g;
       launch_mfma_multi(precision, g.d.nm, g.nac, g.nwa, g.ny, (unsigned)grid, d_params, ml, st);
@@ -242,13 +268,18 @@ int enqueue_tick_forked(int precision, std::vector<TickWork>& work, const SweepP
   if (n < 2) return enqueue_tick(precision, work, d_params, st);
   (void)hipEventRecord(fork, st);
   for (int c = 0; c < n; ++c) (void)hipStreamWaitEvent(cs[c], fork, 0);
-  int nl = 0, c = 0;
-  for (auto& kv : by_variant) nl += enqueue_tick(precision, kv.second, d_params, cs[c++ % n]);
+  int nl = 0, c = 0, err = 0;
+  for (auto& kv : by_variant) {
+    const int k = enqueue_tick(precision, kv.second, d_params, cs[c++ % n]);
+    if (k < 0) err = k;
+    else nl += k;
+  }
+  // joined also after a failed assembly, so that no stream is left forked
   for (int k = 0; k < n; ++k) {
     (void)hipEventRecord(join[k], cs[k]);
     (void)hipStreamWaitEvent(st, join[k], 0);
   }
-  return nl;
+  return err ? err : nl;
 }
 
 // Boundary fill of the face planes of several problems (before their first
@@ -327,11 +358,19 @@ struct MswBatch {
   // and joined back to the run's stream (also inside the captured graph)
   hipStream_t chain_st[kChainStreams] = {};
   hipEvent_t fork_ev = nullptr, join_ev[kChainStreams] = {};
+  // recorded after every run on the run's stream: the graph, the arena and
+  // the parameters of the last build are released only after it
+  hipEvent_t done_ev = nullptr;
+  bool ran = false;
 };
 
 namespace {
 
+// Drop the built run.  Waits for the last run first: a graph exec or the
+// memory its kernels use must not go away while a replay may still run.
 void batch_release(MswBatch* b) {
+  if (b->ran && b->done_ev) (void)hipEventSynchronize(b->done_ev);
+  b->ran = false;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
@@ -415,29 +454,26 @@ int batch_enqueue(MswBatch* b, hipStream_t st) {
   for (size_t i = 0; i < b->probs.size(); ++i) chains[b->probs[i].g.variant()].push_back((int)i);
   const int nch = std::min<int>((int)chains.size(), kChainStreams);
   if (nch > 1) {
-    for (int c = 0; c < nch; ++c) {
-      if (!b->chain_st[c])
-        HIP_OK(hipStreamCreateWithFlags(&b->chain_st[c], hipStreamNonBlocking), "chain stream");
-      if (!b->join_ev[c])
-        HIP_OK(hipEventCreateWithFlags(&b->join_ev[c], hipEventDisableTiming), "chain event");
-    }
-    if (!b->fork_ev)
-      HIP_OK(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming), "chain event");
+    // the chain streams and events exist from msw_batch_create on: nothing
+    // is created while a capture is open
     HIP_OK(hipEventRecord(b->fork_ev, st), "fork");
     for (int c = 0; c < nch; ++c) HIP_OK(hipStreamWaitEvent(b->chain_st[c], b->fork_ev, 0), "fork");
   }
-  int ci = 0;
+  int ci = 0, err = 0;
   for (auto& kv : chains) {
     hipStream_t cs = nch > 1 ? b->chain_st[ci % nch] : st;
     ++ci;
-    for (int t = 0; t < nt; ++t) {
+    for (int t = 0; t < nt && !err; ++t) {
       std::vector<TickWork> work;
       for (int i : kv.second) {
         const BatchProb& p = b->probs[i];
         const int l = t - p.start_tick;
         if (l >= 0 && l < p.s->nlaunch()) work.push_back(TickWork{i, &p.g, p.s, l});
       }
-      if (!work.empty()) nk += enqueue_tick(prec, work, b->d_params, cs);
+      if (work.empty()) continue;
+      const int k = enqueue_tick(prec, work, b->d_params, cs);
+      if (k < 0) err = -k;
+      else nk += k;
     }
   }
   if (nch > 1) {
@@ -446,6 +482,7 @@ int batch_enqueue(MswBatch* b, hipStream_t st) {
       HIP_OK(hipStreamWaitEvent(st, b->join_ev[c], 0), "join");
     }
   }
+  if (err) return err;  // the chains are joined: a capture can still end
   b->nticks = nt;
   b->nkernels = nk;
   HIP_OK(hipGetLastError(), "batch launch");
@@ -504,7 +541,6 @@ int batch_build(MswBatch* b) {
   HIP_OK(hipMemcpy(b->d_params, hp.data(), sizeof(SweepParams) * hp.size(), hipMemcpyHostToDevice),
          "batch params upload");
   if (b->opt.use_graph) {
-    if (!b->cap) HIP_OK(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking), "capture stream");
     HIP_OK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeRelaxed), "begin capture");
     const int rc = batch_enqueue(b, b->cap);
     hipGraph_t g = nullptr;
@@ -515,7 +551,7 @@ int batch_build(MswBatch* b) {
     }
     HIP_OK(e, "end capture");
     b->graph = g;
-    HIP_OK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph instantiate");
+    HIP_OK(hipGraphInstantiateWithFlags(&b->exec, b->graph, 0), "graph instantiate");
   }
   b->built = true;
   return MSW_OK;
@@ -528,9 +564,13 @@ int batch_run(MswBatch* b, hipStream_t st) {
   }
   if (b->opt.use_graph) {
     HIP_OK(hipGraphLaunch(b->exec, st), "graph launch");
-    return MSW_OK;
+  } else {
+    const int rc = batch_enqueue(b, st);
+    if (rc) return rc;
   }
-  return batch_enqueue(b, st);
+  HIP_OK(hipEventRecord(b->done_ev, st), "run event");
+  b->ran = true;
+  return MSW_OK;
 }
 
 // Mean milliseconds of one run over reps (after one untimed run).
@@ -558,7 +598,7 @@ extern "C" {
 
 void msw_batch_default_options(MswBatchOptions* o) {
   if (!o) return;
-  o->precision = MSW_PREC_MIXED_FP32;
+  o->precision = MSW_PREC_FP64;
   o->use_graph = 1;
   o->schedule_kind = 0;
   o->stagger = 1;
@@ -583,6 +623,17 @@ int msw_batch_create(int nprob, const MswDims* dims, const MswBatchOptions* opt,
   b->opt = o;
   b->dims.assign(dims, dims + nprob);
   b->probs.resize(nprob);
+  hipError_t e = hipEventCreateWithFlags(&b->done_ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming);
+  for (int c = 0; c < kChainStreams && e == hipSuccess; ++c) {
+    e = hipStreamCreateWithFlags(&b->chain_st[c], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join_ev[c], hipEventDisableTiming);
+  }
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    msw_batch_destroy(b);
+    return hip_fail(e, "batch streams and events");
+  }
   *out = b;
   return MSW_OK;
 }
@@ -756,6 +807,7 @@ int msw_batch_destroy(MswBatch* b) {
     if (b->join_ev[c]) (void)hipEventDestroy(b->join_ev[c]);
   }
   if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
+  if (b->done_ev) (void)hipEventDestroy(b->done_ev);
   delete b;
   return MSW_OK;
 }
@@ -836,6 +888,7 @@ struct Active {
   const double* mf = nullptr;
   int next = 0;  // next launch of its schedule
   bool started = false;
+  int err = MSW_OK;  // a tick of this request failed to assemble
   double t_admit = 0;
 };
 
@@ -1055,8 +1108,15 @@ void server_loop(MswServer* s) {
         if (a.next == a.s->nlaunch()) finishing.push_back((int)k);
       }
       if (!work.empty()) {
-        s->nlaunch += enqueue_tick_forked(s->opt.precision, work, s->d_params, s->st,
+        const int k = enqueue_tick_forked(s->opt.precision, work, s->d_params, s->st,
                                           s->chain_st, s->fork_ev, s->join_ev, kChainStreams);
+        if (k < 0) {  // reported with each request of the tick when it retires
+          for (const TickWork& w : work)
+            for (Active& a : s->active)
+              if (a.slot == w.slot) a.err = -k;
+        } else {
+          s->nlaunch += k;
+        }
         s->nticks++;
       }
       // norms of the problems whose last launch was in this tick, after it
@@ -1076,7 +1136,8 @@ void server_loop(MswServer* s) {
       for (int slot : pending)
         for (size_t k = 0; k < s->active.size(); ++k)
           if (s->active[k].slot == slot) {
-            finish(s, s->active[k], e == hipSuccess ? MSW_OK : MSW_ERR_HIP);
+            finish(s, s->active[k],
+                   s->active[k].err ? s->active[k].err : e == hipSuccess ? MSW_OK : MSW_ERR_HIP);
             s->active.erase(s->active.begin() + k);
             break;
           }
@@ -1099,10 +1160,11 @@ extern "C" {
 
 void msw_server_default_options(MswServerOptions* o) {
   if (!o) return;
-  o->precision = MSW_PREC_MIXED_FP32;
+  o->precision = MSW_PREC_FP64;
   o->hbm_budget_bytes = (size_t)32 << 30;
   o->max_active = 512;
   o->keep_outputs = 0;
+  o->schedule_kind = 0;
 }
 
 int msw_server_tune_from_batch(MswServer* s, const MswBatch* b) {
@@ -1129,8 +1191,11 @@ int msw_server_create(const MswServerOptions* opt, MswServer** out) {
     return set_error(MSW_ERR_INVALID, "unknown precision");
   if (o.max_active < 1 || o.hbm_budget_bytes < kAlign)
     return set_error(MSW_ERR_INVALID, "max_active >= 1 and a nonzero budget are required");
+  if (o.schedule_kind != 0 && o.schedule_kind != 1)
+    return set_error(MSW_ERR_INVALID, "schedule_kind must be 0 or 1");
   MswServer* s = new MswServer();
   s->opt = o;
+  s->schedule_kind = o.schedule_kind;
   auto fail = [&](hipError_t e, const char* what) {
     msw_server_destroy(s);
     return hip_fail(e, what);

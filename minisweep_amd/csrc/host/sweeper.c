@@ -140,6 +140,10 @@ void Sweeper_create(Sweeper* sweeper, Dimensions dims, const Quantities* quan,
   Insist(sweeper->nthread_z > 0 ? "Invalid thread count supplied." : 0);
 
   sweeper->precision = env->precision_;
+  /* z blocks are built into the FP64 sweep only (the mixed forms keep their
+     FP32 running sum for the whole grid); refused rather than ignored */
+  Insist((sweeper->nblock_z == 1 || sweeper->precision == MSW_PREC_FP64)
+             ? "z blocking (--nblock_z > 1) is supported in FP64 only" : 0);
   {
     MswDims d;
     d.ncell_x = dims.ncell_x;

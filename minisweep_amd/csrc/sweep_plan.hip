@@ -605,6 +605,10 @@ struct MswPlan {
   hipStream_t cst = nullptr, cst_out = nullptr;
   hipEvent_t ev_in[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr};
   hipEvent_t ev_out[2] = {nullptr, nullptr};
+  // recorded on the caller's stream at entry: the copy streams start after
+  // everything the caller enqueued before (a previous call's last copy back
+  // included), so calls are ordered as their stream implies
+  hipEvent_t ev_entry = nullptr;
 };
 
 template <int NM, int NAC>
@@ -695,6 +699,7 @@ void msw_plan_default_options(MswPlanOptions* o) {
   o->groups_per_workgroup = 0;
   o->tile_rows = 0;
   o->z_block = 0;
+  o->schedule_kind = 0;
 }
 
 int msw_plan_create(const MswDims* d, int precision, MswPlan** out) {
@@ -769,6 +774,10 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
     delete p;
     return set_error(MSW_ERR_INVALID, "z_block must be >= 0");
   }
+  if (opt->schedule_kind != 0 && opt->schedule_kind != 1) {
+    delete p;
+    return set_err(MSW_ERR_INVALID, "schedule_kind must be 0 or 1 (got %d)", opt->schedule_kind);
+  }
   MswDims dblk = *d;  // the geometry one kernel launch sees
   dblk.ncell_z = p->zb;
   if (p->engine == MSW_ENGINE_MFMA && !mfma_face_offsets_fit(dblk, p->na_pad)) {
@@ -797,11 +806,12 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
   else
     p->lds_bytes = sizeof(double) * ((size_t)p->nwa * 2 * kTile * (p->nac + 1) +
                                      (p->nwa > 1 ? (size_t)p->nwa * d->nm * 64 : 0));
-  auto launches = build_schedule(p->ntx, p->nty, 0);
+  const int kind = opt->schedule_kind;
+  auto launches = build_schedule(p->ntx, p->nty, kind);
   if (p->nblk > 1) {  // two passes of four octants, each over every z block
-    launches = build_schedule(p->ntx, p->nty, 0, 0x0F, 0);
+    launches = build_schedule(p->ntx, p->nty, kind, 0x0F, 0);
     p->pass_split = (int)launches.size();
-    auto up = build_schedule(p->ntx, p->nty, 0, 0xF0, 4);
+    auto up = build_schedule(p->ntx, p->nty, kind, 0xF0, 4);
     launches.insert(launches.end(), up.begin(), up.end());
   }
   // energy groups per workgroup, per launch: streamed while the launch
@@ -927,6 +937,7 @@ int msw_plan_destroy(MswPlan* p) {
     if (p->ev_done[k]) (void)hipEventDestroy(p->ev_done[k]);
     if (p->ev_out[k]) (void)hipEventDestroy(p->ev_out[k]);
   }
+  if (p->ev_entry) (void)hipEventDestroy(p->ev_entry);
   if (p->cst) (void)hipStreamDestroy(p->cst);
   if (p->cst_out) (void)hipStreamDestroy(p->cst_out);
   delete p;
@@ -1017,7 +1028,15 @@ int msw_plan_sweep_host(MswPlan* p, const double* vi_host, double* vo_host,
     }
     HIP_TRY(hipStreamCreateWithFlags(&p->cst, hipStreamNonBlocking));
     HIP_TRY(hipStreamCreateWithFlags(&p->cst_out, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&p->ev_entry, hipEventDisableTiming));
   }
+  // Stream order: the copies run on the plan's own streams, which wait here
+  // for the work already on st (an earlier call's sweeps and copies back,
+  // and whatever produced vi_host), so a call queued right behind another,
+  // or reading the vo_host that one wrote, sees its results.
+  HIP_TRY(hipEventRecord(p->ev_entry, st));
+  HIP_TRY(hipStreamWaitEvent(p->cst, p->ev_entry, 0));
+  HIP_TRY(hipStreamWaitEvent(p->cst_out, p->ev_entry, 0));
   const int n = kNoct * d.nm * d.na;
   hipLaunchKernelGGL(k_pack_matrices, dim3((n + 255) / 256), dim3(256), 0, st,
                      a_from_m, m_from_a, p->at, p->mf, d.nm, d.na, p->na_pad);

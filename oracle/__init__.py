@@ -41,6 +41,7 @@ def lib():
         L.oracle_init_state.argtypes = [OracleDims, dp]
         L.oracle_sweep.argtypes = [OracleDims, dp, dp, dp, dp, ctypes.c_int]
         L.oracle_sweep_tileoctants.argtypes = [OracleDims, dp, dp, dp, dp, ctypes.c_int]
+        L.oracle_sweep_ld.argtypes = [OracleDims, dp, dp, dp, dp, ctypes.c_int]
         L.oracle_sweep_range.argtypes = [OracleDims, dp, dp, dp, dp,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.oracle_norms.argtypes = [OracleDims, dp, dp, dp, dp]
@@ -94,6 +95,17 @@ def sweep_tileoctants(d: OracleDims, a, m, vi, nthreads: int = 0) -> np.ndarray:
     vo = np.zeros_like(vi)
     lib().oracle_sweep_tileoctants(d, _p(np.ascontiguousarray(a)), _p(np.ascontiguousarray(m)),
                                    _p(vi), _p(vo), nthreads)
+    return vo
+
+
+def sweep_ld(d: OracleDims, a, m, vi, nthreads: int = 0) -> np.ndarray:
+    """The sweep evaluated in long double, rounded to double at the end: the
+    near-exact answer against which FP64 evaluations are judged on
+    ill-conditioned inputs."""
+    vi = np.ascontiguousarray(vi, dtype=np.float64)
+    vo = np.zeros_like(vi)
+    lib().oracle_sweep_ld(d, _p(np.ascontiguousarray(a)), _p(np.ascontiguousarray(m)),
+                          _p(vi), _p(vo), nthreads)
     return vo
 
 

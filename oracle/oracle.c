@@ -300,6 +300,110 @@ void oracle_sweep(OracleDims d, const double* A, const double* M,
   oracle_sweep_range(d, A, M, vi, vo, 0, d.ne, nthreads);
 }
 
+/* The sweep of oracle_sweep evaluated in long double (x87 extended: 64-bit
+ * significand), every value rounded to double only at the end.  Same loop
+ * order, boundary faces and solve (sweeper_simple_c.h:110-270,
+ * quantities_testing_kernels.h:380-446); its rounding is 2^-11 of FP64's,
+ * so it stands for the exact sweep when judging how far an FP64 evaluation
+ * is from it on ill-conditioned inputs. */
+typedef long double LD;
+void oracle_sweep_ld(OracleDims d, const double* A, const double* M, const double* vi,
+                     double* vo, int nthreads) {
+  const int nx = d.ncell_x, ny = d.ncell_y, nz = d.ncell_z, na = d.na, nm = d.nm, nu = d.nu;
+  const size_t nfxy = (size_t)nx * ny * na * nu, nfxz = (size_t)nx * nz * na * nu;
+  const size_t nfyz = (size_t)ny * nz * na * nu;
+  const size_t group = (size_t)nx * ny * nm * nu;
+  int ie;
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+  for (ie = 0; ie < d.ne; ++ie) {
+    LD* fxy = (LD*)malloc(sizeof(LD) * nfxy);
+    LD* fxz = (LD*)malloc(sizeof(LD) * nfxz);
+    LD* fyz = (LD*)malloc(sizeof(LD) * nfyz);
+    LD* vs = (LD*)malloc(sizeof(LD) * (size_t)na * nu);
+    LD* acc = (LD*)calloc(group * nz, sizeof(LD)); /* this group's vo, [iz][iy][ix][iu][im] */
+    int octant, ix, iy, iz, ia, iu, im;
+    for (octant = 0; octant < NOCT; ++octant) {
+      const int dx = dir_of(octant, 0), dy = dir_of(octant, 1), dz = dir_of(octant, 2);
+      const LD so = (LD)sf_octant(octant), so_r = 1.0L / so;
+      /* boundary faces one cell outside the domain (sweeper_simple_c.h:124-186) */
+      iz = dz == +1 ? -1 : nz;
+      for (iy = 0; iy < ny; ++iy)
+        for (ix = 0; ix < nx; ++ix)
+          for (iu = 0; iu < nu; ++iu)
+            for (ia = 0; ia < na; ++ia)
+              fxy[ia + na * (iu + nu * (ix + nx * iy))] =
+                  (LD)oracle_face_value(d, ix, iy, iz, ie, ia, iu, octant);
+      iy = dy == +1 ? -1 : ny;
+      for (iz = 0; iz < nz; ++iz)
+        for (ix = 0; ix < nx; ++ix)
+          for (iu = 0; iu < nu; ++iu)
+            for (ia = 0; ia < na; ++ia)
+              fxz[ia + na * (iu + nu * (ix + nx * iz))] =
+                  (LD)oracle_face_value(d, ix, iy, iz, ie, ia, iu, octant);
+      ix = dx == +1 ? -1 : nx;
+      for (iz = 0; iz < nz; ++iz)
+        for (iy = 0; iy < ny; ++iy)
+          for (iu = 0; iu < nu; ++iu)
+            for (ia = 0; ia < na; ++ia)
+              fyz[ia + na * (iu + nu * (iy + ny * iz))] =
+                  (LD)oracle_face_value(d, ix, iy, iz, ie, ia, iu, octant);
+      int cx, cy, cz;
+      for (cz = 0, iz = dz == +1 ? 0 : nz - 1; cz < nz; ++cz, iz += dz)
+        for (cy = 0, iy = dy == +1 ? 0 : ny - 1; cy < ny; ++cy, iy += dy)
+          for (cx = 0, ix = dx == +1 ? 0 : nx - 1; cx < nx; ++cx, ix += dx) {
+            const size_t base = ind_state(&d, ix, iy, iz, ie, 0, 0);
+            LD* out = acc + (size_t)iz * group + ((size_t)iy * nx + ix) * nu * nm;
+            const LD s = (LD)sf_space(ix, iy, iz), s_r = 1.0L / s;
+            const LD sx_r = 1.0L / (LD)sf_space(ix - dx, iy, iz);
+            const LD sy_r = 1.0L / (LD)sf_space(ix, iy - dy, iz);
+            const LD sz_r = 1.0L / (LD)sf_space(ix, iy, iz - dz);
+            for (iu = 0; iu < nu; ++iu)
+              for (ia = 0; ia < na; ++ia) {
+                LD r = 0.0L;
+                for (im = 0; im < nm; ++im)
+                  r += (LD)A[ia + na * (im + nm * octant)] * (LD)vi[base + im + (size_t)nm * iu];
+                vs[ia + na * iu] = r;
+              }
+            for (ia = 0; ia < na; ++ia) {
+              const LD wz = 0.25L - 1.0L / (LD)sf_angle(ia);
+              for (iu = 0; iu < nu; ++iu) {
+                LD* f1 = &fxy[ia + na * (iu + nu * (ix + nx * iy))];
+                LD* f2 = &fxz[ia + na * (iu + nu * (ix + nx * iz))];
+                LD* f3 = &fyz[ia + na * (iu + nu * (iy + ny * iz))];
+                const LD result = (vs[ia + na * iu] * s_r +
+                                   (*f1 * 0.5L * sz_r + *f2 * 0.25L * sy_r + *f3 * wz * sx_r) *
+                                       so_r) * s;
+                vs[ia + na * iu] = result;
+                *f1 = *f2 = *f3 = result * so;
+              }
+            }
+            for (iu = 0; iu < nu; ++iu)
+              for (im = 0; im < nm; ++im) {
+                LD r = 0.0L;
+                for (ia = 0; ia < na; ++ia)
+                  r += (LD)M[im + nm * (ia + na * octant)] * vs[ia + na * iu];
+                out[im + (size_t)nm * iu] += r;
+              }
+          }
+    }
+    for (iz = 0; iz < nz; ++iz) {
+      const LD* src = acc + (size_t)iz * group;
+      double* dst = &vo[ind_state(&d, 0, 0, iz, ie, 0, 0)];
+      size_t k;
+      for (k = 0; k < group; ++k) dst[k] = (double)src[k];
+    }
+    free(fxy);
+    free(fxz);
+    free(fyz);
+    free(vs);
+    free(acc);
+  }
+  (void)nthreads;
+}
+
 /* The reference's tiled-octant ordering (sweeper_tileoctants_c.h:127-189):
  * 8 tile steps; at step t octant o sweeps the half-domain box whose half
  * along axis b is the lower one when (bit b of t is 0) equals (o runs upward

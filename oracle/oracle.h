@@ -56,6 +56,12 @@ void oracle_sweep(OracleDims d, const double* a_from_m, const double* m_from_a,
 void oracle_sweep_tileoctants(OracleDims d, const double* a_from_m, const double* m_from_a,
                               const double* vi, double* vo, int nthreads);
 
+/* The same sweep evaluated in long double (64-bit significand), rounded to
+ * double at the end: the reference point for ill-conditioned inputs, where
+ * any FP64 evaluation (the reference's included) is 1e-12-level off. */
+void oracle_sweep_ld(OracleDims d, const double* a_from_m, const double* m_from_a,
+                     const double* vi, double* vo, int nthreads);
+
 /* Same sweep restricted to the energy groups [ie_begin, ie_end): used to time
  * a bounded sample of a large problem. */
 void oracle_sweep_range(OracleDims d, const double* a_from_m,

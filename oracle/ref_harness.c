@@ -12,7 +12,8 @@
  *
  * usage: ref_harness "<argstring>" <seed> <perturb> <out.bin>
  *   seed    : 64-bit generator seed
- *   perturb : 0 = manufactured vi, 1 = vi*(1+u/4), 2 = vi = u, u in [-1,1)
+ *   perturb : 0 = manufactured vi, 1 = vi*(1+u/4), 2 = vi = u,
+ *             3 = vi*(1+u*2^-8), u in [-1,1)
  * out.bin   : int32 header [ncx ncy ncz ne nm na nu niter] then
  *             double normsq, normsqdiff, then vi0 (input), vi, vo (final).
  */
@@ -82,6 +83,7 @@ int main(int argc, char** argv) {
   for (i = 0; i < n; ++i) {
     if (perturb == 1) Pointer_h(&vi)[i] *= 1.0 + 0.25 * rng_uniform();
     if (perturb == 2) Pointer_h(&vi)[i] = rng_uniform();
+    if (perturb == 3) Pointer_h(&vi)[i] *= 1.0 + rng_uniform() / 256.0;
   }
   double* vi0 = (double*)malloc(n * sizeof(double));
   for (i = 0; i < n; ++i) vi0[i] = Pointer_h(&vi)[i];

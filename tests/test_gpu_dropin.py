@@ -59,6 +59,8 @@ def test_sweep_result_line(args, shape, cuda):
     (["--nsemiblock", 3], "Invalid semiblock count supplied"),
     (["--bogus", 1], "Invalid argument detected."),
     (["--nm", 5], "nm must be 4, 8 or 16"),
+    (["--ncell_z", 4, "--nblock_z", 2, "--precision", 1],
+     "z blocking (--nblock_z > 1) is supported in FP64 only"),
 ])
 def test_sweep_config_errors(args, msg, cuda):
     r = run("sweep", *args)
@@ -66,11 +68,14 @@ def test_sweep_config_errors(args, msg, cuda):
     assert "Insist error" in r.stderr and msg in r.stderr
 
 
-@pytest.mark.parametrize("battery", ["serial", "mpi", "variants", "tasks"])
+# pairs per battery: serial 54, openmp 981 (tester.c:103-211), tasks 64,
+# cuda 984 (tester.c:290-390), mpi 4 one-process pairs, variants 2
+@pytest.mark.parametrize("battery", ["serial", "openmp", "tasks", "cuda", "mpi", "variants"])
 def test_tester_battery(battery, cuda):
     """The reference's compare_runs batteries (tester.c): every pair must
     give normsqdiff == 0 and identical normsq, exactly."""
     r = run("tester", "--battery", battery, timeout=110)
+    print(r.stdout.strip().splitlines()[-1] if r.stdout.strip() else "(no output)", flush=True)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     m = re.search(r"TESTS (\d+)    PASSED (\d+)    FAILED (\d+)", r.stdout)
     assert m and int(m.group(1)) > 0 and m.group(1) == m.group(2) and m.group(3) == "0"

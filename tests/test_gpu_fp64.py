@@ -196,3 +196,33 @@ def test_tile_rows_vs_oracle(shape, cuda):
                 assert np.max(np.abs(o - outs[0])) <= 1e-5 * np.max(np.abs(outs[0]))
             else:
                 assert_close(o, outs[0], what="tile heights")
+
+
+ILL_SHAPES = [(9, 13, 9, 8, 4, 32, 4), (12, 10, 11, 2, 16, 64, 4), (17, 3, 5, 1, 8, 40, 2)]
+
+
+@pytest.mark.parametrize("shape", ILL_SHAPES, ids=[str(s) for s in ILL_SHAPES])
+def test_ill_conditioned_input_vs_long_double(shape, cuda):
+    """The manufactured state perturbed by ±2^-8: the sweep amplifies the
+    perturbation ~400x through the cancelling rows of A and M, and the
+    reference's FP64 formula (the oracle) lands 1e-12-level per element from
+    the exact answer (the long-double evaluation).  The GPU sweep must be at
+    least as close to it: per element and normwise within twice the oracle's
+    own FP64 error, or 1e-12, whichever is larger."""
+    import torch
+    from parity import normwise_err
+    print(f"case ill-conditioned {shape}", flush=True)
+    d = msw.dims(*shape)
+    od = oracle.dims(*shape)
+    base = oracle.init_state(od)
+    x = base * (1.0 + np.random.default_rng(19).uniform(-2.0 ** -8, 2.0 ** -8, base.size))
+    _, got, _, _ = _run_gpu(d, x, 1, torch, cuda)
+    a, m = oracle.init_matrices(od)
+    ld = oracle.sweep_ld(od, a, m, x)
+    f64 = oracle.sweep(od, a, m, x)
+    e_gpu, e_ref = elem_err(got, ld), elem_err(f64, ld)
+    n_gpu, n_ref = normwise_err(got, ld), normwise_err(f64, ld)
+    print(f"  vs long double: GPU elem {e_gpu:.3e} normwise {n_gpu:.3e}; "
+          f"oracle elem {e_ref:.3e} normwise {n_ref:.3e}", flush=True)
+    assert e_gpu <= max(2 * e_ref, 1e-12)
+    assert n_gpu <= max(2 * n_ref, 1e-14)

@@ -116,3 +116,54 @@ def test_z_block_options_checked(cuda):
         msw.Plan(d, msw.PREC_MIXED_FP32, engine=msw.ENGINE_MFMA, z_block=4)
     with pytest.raises(RuntimeError):
         msw.Plan(d, msw.PREC_FP64, engine=msw.ENGINE_VALU, z_block=4)
+
+
+@pytest.mark.parametrize("zb", [2, 4])
+def test_host_sweep_matches_oracle(zb, cuda):
+    """Out-of-core sweep of a perturbed input against the CPU oracle."""
+    import torch
+    print(f"case host sweep vs oracle zb {zb}", flush=True)
+    shape = (13, 10, 8, 3, 16, 32, 2)
+    d = msw.dims(*shape)
+    od = oracle.dims(*shape)
+    vi0 = _perturbed(shape, 17)
+    ta, tm = _mats(torch, cuda, d)
+    plan = msw.Plan(d, msw.PREC_FP64, engine=msw.ENGINE_MFMA, z_block=zb)
+    hvi = torch.from_numpy(vi0).pin_memory()
+    hvo = torch.full((vi0.size,), float("nan"), dtype=torch.float64).pin_memory()
+    plan.sweep_host(hvi, hvo, ta, tm)
+    torch.cuda.synchronize()
+    plan.close()
+    assert_close(hvo.numpy(), oracle.sweep(od, *oracle.init_matrices(od), vi0),
+                 what=f"host sweep zb {zb} vs oracle")
+
+
+def test_host_sweeps_ping_pong_without_sync(cuda):
+    """Two out-of-core sweeps queued back to back on one stream, the second
+    reading the first's output (the reference runner's iteration pattern),
+    with no synchronization between them: the same bits as two device
+    sweeps.  The second call's copies must wait for the first call's last
+    copy back."""
+    import torch
+    print("case host sweeps ping-pong without sync", flush=True)
+    shape = (13, 10, 8, 3, 16, 32, 2)
+    d = msw.dims(*shape)
+    vi0 = _perturbed(shape, 23)
+    ta, tm = _mats(torch, cuda, d)
+    plan = msw.Plan(d, msw.PREC_FP64, engine=msw.ENGINE_MFMA, z_block=2)
+    a = torch.from_numpy(vi0).to(cuda)
+    b = torch.empty_like(a)
+    c = torch.empty_like(a)
+    plan.sweep(a, b, ta, tm)
+    plan.sweep(b, c, ta, tm)
+    torch.cuda.synchronize()
+    dev1, dev2 = b.cpu().numpy(), c.cpu().numpy()
+    h0 = torch.from_numpy(vi0).pin_memory()
+    h1 = torch.full((vi0.size,), float("nan"), dtype=torch.float64).pin_memory()
+    h2 = torch.full((vi0.size,), float("nan"), dtype=torch.float64).pin_memory()
+    plan.sweep_host(h0, h1, ta, tm)
+    plan.sweep_host(h1, h2, ta, tm)
+    torch.cuda.synchronize()
+    plan.close()
+    np.testing.assert_array_equal(h1.numpy(), dev1)
+    np.testing.assert_array_equal(h2.numpy(), dev2)

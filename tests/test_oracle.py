@@ -164,3 +164,57 @@ def test_oracle_tileoctants_ordering(shape):
     assert_close(oracle.sweep_tileoctants(d, a, m, vi), oracle.sweep(d, a, m, vi),
                  what="tiled octants vs octant order")
     np.testing.assert_array_equal(oracle.sweep_tileoctants(d, a, m, base), base)
+
+
+# ---- the long-double evaluation (oracle.sweep_ld) ---------------------------
+
+LD_SHAPES = [(3, 4, 2, 2, 16, 8, 4), (9, 13, 9, 2, 4, 32, 4), (6, 5, 7, 1, 16, 32, 2)]
+
+
+def _perturbed(d, amp, seed):
+    base = oracle.init_state(d)
+    return base * (1.0 + amp * np.random.default_rng(seed).uniform(-1, 1, base.size))
+
+
+@pytest.mark.parametrize("shape", LD_SHAPES, ids=[str(s) for s in LD_SHAPES])
+def test_oracle_ld_pinned(shape):
+    """The long-double sweep is the same sweep: exact on the manufactured
+    input, and within 1e-13 of the FP64 oracle on a well-conditioned input
+    (±1/4 perturbation, where FP64 rounding stays ~1e-14)."""
+    d = oracle.dims(*shape)
+    a, m = oracle.init_matrices(d)
+    vi = oracle.init_state(d)
+    np.testing.assert_array_equal(oracle.sweep_ld(d, a, m, vi), vi)
+    x = _perturbed(d, 0.25, 3)
+    assert_close(oracle.sweep_ld(d, a, m, x), oracle.sweep(d, a, m, x), tol=1e-13,
+                 what="long double vs FP64 oracle, well-conditioned")
+
+
+@pytest.mark.parametrize("path", [p for p in GOLDEN if "_p1" in p],
+                         ids=[os.path.basename(p)[:-4] for p in GOLDEN if "_p1" in p])
+def test_oracle_ld_matches_reference_golden(path):
+    """One sweep of the compiled reference's perturbed golden inputs."""
+    g = np.load(path)
+    ncx, ncy, ncz, ne, nm, na, nu, nit = (int(x) for x in g["dims"])
+    if nit != 1:
+        pytest.skip("single-sweep fixtures only")
+    d = oracle.dims(ncx, ncy, ncz, ne, nm, na, nu)
+    a, m = oracle.init_matrices(d)
+    assert_close(oracle.sweep_ld(d, a, m, g["vi_in"]), g["vo"], what="long double vs reference")
+
+
+def test_fp64_error_on_ill_conditioned_input():
+    """Why the FP64 1e-12 per-element bar is checked on ±1/4 perturbations:
+    on the manufactured state perturbed by ±2^-8 the sweep is ill-conditioned
+    (the second-difference rows of A and M cancel the affine part, leaving the
+    perturbation amplified ~400x), and the reference's own FP64 formula -- the
+    oracle -- is more than 1e-12 per element from the long-double answer, while
+    normwise it stays near 1e-13."""
+    d = oracle.dims(9, 13, 9, 8, 4, 32, 4)
+    a, m = oracle.init_matrices(d)
+    x = _perturbed(d, 2.0 ** -8, 11)
+    ld = oracle.sweep_ld(d, a, m, x)
+    f64 = oracle.sweep(d, a, m, x)
+    from parity import elem_err, normwise_err
+    assert elem_err(f64, ld) > 1e-12
+    assert normwise_err(f64, ld) < 2e-13
