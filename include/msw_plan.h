@@ -145,6 +145,16 @@ int msw_plan_destroy(MswPlan* plan);
 int msw_state_norms(const MswDims* d, const double* vi, const double* vo,
                     double* out2, void* stream);
 
+/* msw_state_norms without any allocation in the call (no counterpart in the
+ * reference): for callers that cannot allocate device memory, such as code
+ * under HIP graph capture or a loop that forbids hipMalloc.  Same kernels,
+ * same result bits as msw_state_norms.  The block partials (16 KiB) live in
+ * a static device array of the library, one per device, held until the
+ * process ends; calls that overlap on one device race on it.  Any valid
+ * state size fits (the partials' count does not depend on it). */
+int msw_state_norms_noalloc(const MswDims* d, const double* vi, const double* vo,
+                            double* out2, void* stream);
+
 /* Fill v with the manufactured input state (Quantities_init_state,
  * src/3_sweeper/quantities_testing.h:60-90) on the device. */
 int msw_init_state(const MswDims* d, double* v, void* stream);

@@ -97,6 +97,7 @@ def lib():
         L.msw_plan_last_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_float),
                                             ctypes.c_int]
         L.msw_state_norms.argtypes = [dims_p, vp, vp, vp, vp]
+        L.msw_state_norms_noalloc.argtypes = [dims_p, vp, vp, vp, vp]
         L.msw_init_state.argtypes = [dims_p, vp, vp]
         L.msw_init_matrices_host.argtypes = [ctypes.c_int, ctypes.c_int,
                                              ctypes.POINTER(ctypes.c_double),
@@ -252,6 +253,13 @@ def state_norms(d: MswDims, vi, vo, stream=None):
                                  _stream_ptr(stream)), "msw_state_norms")
     o = out.cpu()
     return float(o[0]), float(o[1])
+
+
+def state_norms_noalloc(d: MswDims, vi, vo, out, stream=None):
+    """(normsq, normsqdiff) written into out (2 float64 on the device) with
+    no allocation in the library call (msw_state_norms_noalloc)."""
+    _check(lib().msw_state_norms_noalloc(ctypes.byref(d), _dptr(vi), _dptr(vo), _dptr(out),
+                                         _stream_ptr(stream)), "msw_state_norms_noalloc")
 
 
 def init_state(d: MswDims, v, stream=None):

@@ -31,6 +31,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1143,6 +1144,64 @@ int msw_state_norms(const MswDims* d, const double* vi, const double* vo,
   // share it, so they must not overlap
   static thread_local double* part = nullptr;
   if (!part) HIP_TRY(hipMalloc(&part, sizeof(double) * 2 * kNormBlocks));
+  hipLaunchKernelGGL(k_norms_partial, dim3(kNormBlocks), dim3(kNormThreads), 0, st,
+                     vi, vo, msw_size_state(d), part);
+  hipLaunchKernelGGL(k_norms_final, dim3(1), dim3(kNormThreads), 0, st, part, out2);
+  HIP_TRY(hipGetLastError());
+  return MSW_OK;
+}
+
+}  // extern "C"
+
+// ---- msw_state_norms_noalloc: the block partials in a static device array --
+// One copy of `scratch` per device, held until the process ends.  Host code
+// never passes `scratch` itself to a kernel (on the host it names a stand-in,
+// not device memory): each device's address comes from hipGetSymbolAddress,
+// taken once per device with that device current, and only that address is
+// used.  Every call on a device shares its copy, so calls that overlap on one
+// device race; nothing in it needs clearing (k_norms_partial writes every
+// partial before k_norms_final reads them).
+namespace msw {
+constexpr size_t kScratchDoubles = 2 * (size_t)kNormBlocks;  // 16 KiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[kScratchDevices] = {};
+
+static int scratch_for_current_device(double** out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices)
+    return set_err(MSW_ERR_UNSUPPORTED, "device %d: scratch kept for devices 0..%d only", dev,
+                   kScratchDevices - 1);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return set_err(MSW_ERR_HIP, "scratch address on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_dev[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_dev[dev];
+  return MSW_OK;
+}
+}  // namespace msw
+
+extern "C" {
+
+int msw_state_norms_noalloc(const MswDims* d, const double* vi, const double* vo, double* out2,
+                            void* stream) {
+  int rc = check_dims(d);
+  if (rc) return rc;
+  if (!vi || !vo || !out2) return set_err(MSW_ERR_INVALID, "null norms argument");
+  // the partials' count is fixed (kNormBlocks per norm) whatever the state
+  // size, so every valid state fits; the check guards the constants
+  if (norms_scratch_doubles() > kScratchDoubles)
+    return set_err(MSW_ERR_UNSUPPORTED, "norm partials (%zu doubles) exceed the scratch (%zu)",
+                   norms_scratch_doubles(), kScratchDoubles);
+  double* part = nullptr;
+  if ((rc = scratch_for_current_device(&part))) return rc;
+  hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_norms_partial, dim3(kNormBlocks), dim3(kNormThreads), 0, st,
                      vi, vo, msw_size_state(d), part);
   hipLaunchKernelGGL(k_norms_final, dim3(1), dim3(kNormThreads), 0, st, part, out2);
