@@ -109,6 +109,8 @@ int check_dims_mfma(const MswDims& d) {
     return set_error(MSW_ERR_INVALID, "all dimensions must be positive");
   if (d.nm != 4 && d.nm != 8 && d.nm != 16)
     return set_error(MSW_ERR_UNSUPPORTED, "nm must be 4, 8 or 16");
+  if ((size_t)d.ncell_x * d.ncell_y * d.ncell_z * d.ne * d.nu >= (1ull << 32))
+    return set_error(MSW_ERR_UNSUPPORTED, "state of 2^32 or more cells x unknowns");
   return MSW_OK;
 }
 
@@ -134,18 +136,27 @@ int make_geom(const MswDims& d, int nac, Geom* g, int ny = 0, int precision = MS
   return MSW_OK;
 }
 
-// FP32 running vo sum of the mixed forms (none in FP64), followed in the
-// BF16 form by the BF16 copy of vi.
+// Mixed forms (none in FP64): the FP32 running vo sum, then the operand
+// copies of vi -- the BF16 part (both forms), then the FP32 part (FP32 form).
 size_t vo32_bytes(const Geom& g, int precision) {
-  const size_t one = align_up(sizeof(float) * g.state_doubles());
-  return precision == MSW_PREC_FP64         ? 0
-         : precision == MSW_PREC_MIXED_BF16 ? one + align_up(sizeof(short) * g.state_doubles())
-                                            : one;
+  if (precision == MSW_PREC_FP64) return 0;
+  const size_t n = g.state_doubles();
+  return align_up(sizeof(float) * n) + align_up(sizeof(unsigned short) * n) +
+         (precision == MSW_PREC_MIXED_FP32 ? align_up(sizeof(float) * n) : 0);
 }
-const unsigned short* vib_of(const Geom& g, float* vo32, int precision) {
-  return vo32 && precision == MSW_PREC_MIXED_BF16
-             ? (const unsigned short*)((char*)vo32 + align_up(sizeof(float) * g.state_doubles()))
+unsigned short* vlo_of(const Geom& g, float* vo32) {
+  return vo32 ? (unsigned short*)((char*)vo32 + align_up(sizeof(float) * g.state_doubles()))
+              : nullptr;
+}
+float* vhi_of(const Geom& g, float* vo32, int precision) {
+  return vo32 && precision == MSW_PREC_MIXED_FP32
+             ? (float*)((char*)vlo_of(g, vo32) + align_up(sizeof(unsigned short) * g.state_doubles()))
              : nullptr;
+}
+// The mixed forms' operand copies of one problem's vi (nothing in FP64).
+void enqueue_operand_copy(const Geom& g, const double* vi, float* vo32, int precision,
+                          hipStream_t st) {
+  if (vo32) launch_operand_copy(precision, g.d, vi, vhi_of(g, vo32, precision), vlo_of(g, vo32), st);
 }
 
 SweepParams make_params(const Geom& g, const double* vi, double* vo, const double* at,
@@ -153,7 +164,9 @@ SweepParams make_params(const Geom& g, const double* vi, double* vo, const doubl
                         const DevSchedule* s, int precision) {
   SweepParams sp;
   sp.vo32 = vo32;
-  sp.vib = vib_of(g, vo32, precision);
+  sp.vhi = vhi_of(g, vo32, precision);
+  sp.vlo = vlo_of(g, vo32);
+  sp.vib = sp.vlo;
   sp.vi = vi;
   sp.vo = vo;
   sp.at = at;
@@ -282,29 +295,6 @@ int enqueue_tick_forked(int precision, std::vector<TickWork>& work, const SweepP
   return err ? err : nl;
 }
 
-// Boundary fill of the face planes of several problems (before their first
-// tile launch).  Returns the number of launches.
-int enqueue_fill(int precision, const std::vector<std::pair<int, const SweepParams*>>& ps,
-                 const SweepParams* d_params, hipStream_t st) {
-  int nl = 0;
-  for (size_t b = 0; b < ps.size(); b += kMaxSegments) {
-    MultiLaunch ml;
-    ml.nseg = 0;
-    long long grid = 0;
-    for (size_t k = b; k < ps.size() && k < b + kMaxSegments; ++k) {
-      ml.start[ml.nseg] = (int)grid;
-      ml.prob[ml.nseg] = ps[k].first;
-      ml.item_base[ml.nseg] = 0;
-      ml.epw[ml.nseg] = 1;
-      ml.nseg++;
-      grid += mfma_fill_rows(*ps[k].second);
-    }
-    launch_mfma_fill_multi(precision, (unsigned)grid, d_params, ml, st);
-    ++nl;
-  }
-  return nl;
-}
-
 }  // namespace
 
 }  // namespace msw
@@ -430,26 +420,18 @@ int batch_enqueue(MswBatch* b, hipStream_t st) {
   for (auto& p : b->probs) {
     launch_pack(p.afm, p.mfa, p.at, p.mf, p.g.d.nm, p.g.d.na, p.g.nap, st);
     ++nk;
-    if (const unsigned short* vb = vib_of(p.g, p.vo32, b->opt.precision)) {
-      launch_to_bf16(p.vi, (unsigned short*)vb, p.g.state_doubles(), st);
+    if (p.vo32) {
+      enqueue_operand_copy(p.g, p.vi, p.vo32, prec, st);
       ++nk;
     }
   }
-  std::vector<SweepParams> hp(b->probs.size());
-  std::vector<std::pair<int, const SweepParams*>> fill;
-  for (size_t i = 0; i < b->probs.size(); ++i) {
-    const BatchProb& p = b->probs[i];
-    hp[i] = make_params(p.g, p.vi, p.vo, p.at, p.mf, p.fyz, p.fxz, p.vo32, p.s,
-                        b->opt.precision);
-  }
-  for (size_t i = 0; i < b->probs.size(); ++i) fill.emplace_back((int)i, &hp[i]);
-  nk += enqueue_fill(prec, fill, b->d_params, st);
   int nt = 0;
   for (auto& p : b->probs) nt = std::max(nt, p.start_tick + p.s->nlaunch());
   // One chain of ticks per kernel variant: problems of different variants
   // share no launch, so the chains run concurrently on their own streams
-  // (round-robin over kChainStreams), forked after the fill and joined at the
-  // end.  Within a chain, tick t+1 follows tick t.
+  // (round-robin over kChainStreams), forked after the matrix packing and
+  // operand copies and joined at the end.  Within a chain, tick t+1 follows
+  // tick t.
   std::map<std::tuple<int, int, int, int>, std::vector<int>> chains;
   for (size_t i = 0; i < b->probs.size(); ++i) chains[b->probs[i].g.variant()].push_back((int)i);
   const int nch = std::min<int>((int)chains.size(), kChainStreams);
@@ -502,7 +484,7 @@ int batch_build(MswBatch* b) {
     int rc = make_geom(b->dims[i], nac_for(b, b->dims[i]), &p.g, b->row_groups, b->opt.precision);
     if (rc) return rc;
     const int nz = p.g.d.ncell_z;
-    p.g.epw = std::max(1, std::min(p.g.d.ne, (b->stream_target + nz / 2) / nz));
+    p.g.epw = std::max(1, std::min({p.g.d.ne, (b->stream_target + nz / 2) / nz, kMaxStreamGroups}));
     rc = get_schedule(b->scheds, p.g.ntx, p.g.nty, b->opt.schedule_kind, &p.s);
     if (rc) return rc;
     offs[i] = total;
@@ -1026,7 +1008,8 @@ void server_loop(MswServer* s) {
                            &a.g, s->row_groups, s->opt.precision);
         if (!rc) {
           const int nz = a.g.d.ncell_z;
-          a.g.epw = std::max(1, std::min(a.g.d.ne, (s->stream_target + nz / 2) / nz));
+          a.g.epw =
+              std::max(1, std::min({a.g.d.ne, (s->stream_target + nz / 2) / nz, kMaxStreamGroups}));
           rc = get_schedule(s->scheds, a.g.ntx, a.g.nty, s->schedule_kind, &a.s);
         }
         if (!rc) rc = server_matrices(s, a.g, &a.at, &a.mf);
@@ -1081,7 +1064,6 @@ void server_loop(MswServer* s) {
     // ---- enqueue this tick (the active list is only changed by this thread)
     std::vector<int> finishing;
     if (!s->active.empty()) {
-      std::vector<std::pair<int, const SweepParams*>> fill;
       for (size_t k : fresh) {
         Active& a = s->active[k];
         s->h_params[a.slot] =
@@ -1093,12 +1075,9 @@ void server_loop(MswServer* s) {
                                hipMemcpyHostToDevice, s->st);
         else
           launch_init_state(a.g.d, a.vi, s->st);
-        if (const unsigned short* vb = vib_of(a.g, a.vo32, s->opt.precision))
-          launch_to_bf16(a.vi, (unsigned short*)vb, a.g.state_doubles(), s->st);
-        fill.emplace_back(a.slot, &s->h_params[a.slot]);
+        enqueue_operand_copy(a.g, a.vi, a.vo32, s->opt.precision, s->st);
         a.started = true;
       }
-      if (!fill.empty()) s->nlaunch += enqueue_fill(s->opt.precision, fill, s->d_params, s->st);
       std::vector<TickWork> work;
       for (size_t k = 0; k < s->active.size(); ++k) {
         Active& a = s->active[k];

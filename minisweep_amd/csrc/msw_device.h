@@ -31,12 +31,14 @@ struct TileItem {
 __host__ __device__ __forceinline__ int sf_space_exp(int x, int y, int z) {
   return (x + 3 * y + 7 * z + 2) & 3;
 }
-__host__ __device__ __forceinline__ int sf_energy(int ie) {
-  return 1 << (((ie * 1366 + 150889) % 714025) & 3);
+__host__ __device__ __forceinline__ int sf_energy_exp(int ie) {
+  return ((ie * 1366 + 150889) % 714025) & 3;
 }
-__host__ __device__ __forceinline__ int sf_unknown(int iu) {
-  return 1 << (((iu * 741 + 66037) % 312500) & 3);
+__host__ __device__ __forceinline__ int sf_unknown_exp(int iu) {
+  return ((iu * 741 + 66037) % 312500) & 3;
 }
+__host__ __device__ __forceinline__ int sf_energy(int ie) { return 1 << sf_energy_exp(ie); }
+__host__ __device__ __forceinline__ int sf_unknown(int iu) { return 1 << sf_unknown_exp(iu); }
 
 // The cell solve.  Contraction is off: the reference rounds each product and
 // sum separately, and its exact manufactured answer depends on that.
@@ -73,8 +75,14 @@ struct SweepParams {
                      //   MFMA [8][nu][ny][ne][nz][na] (octant-frame z)
   void* fxz;         // y-direction faces: the same with nx for ny
   float* vo32;       // mixed forms: FP32 running sum of vo over the octants (state layout)
-  const unsigned short* vib;  // BF16 form: vi rounded to BF16 (through FP32) once per
-                              // sweep, state layout
+  // Operand copies of vi made once per sweep by the mixed forms, state
+  // layout, each value divided by its cell's scale factor K = S E U (spatial,
+  // group and unknown factors; a power of two, so the division is exact):
+  //   FP32 form: vhi = the FP32 rounding, vlo = the BF16 rounding of the rest
+  //   BF16 form: vib = the BF16 rounding (through FP32)
+  const float* vhi;
+  const unsigned short* vlo;
+  const unsigned short* vib;
   const TileItem* items;
   int nx, ny, nz, ne, na, nu;
   int na_pad;  // na rounded up to whole angle chunks (matrix/face stride)

@@ -26,22 +26,33 @@ struct LaunchSpec {
 
 // Sweep with m2a/a2m on MFMA (sweep_mfma.hip), in the plan's precision
 // (MSW_PREC_*).  nac is 16, 32 or 64; nm 4, 8 or 16; (nwa, ny) a shape for
-// which mfma_shape_supported is true.
+// which mfma_shape_supported is true.  The mixed forms read the operand
+// copies of vi (launch_operand_copy) made for the sweep.
 void launch_mfma(int precision, const LaunchSpec& ls, const SweepParams& sp, hipStream_t st);
 bool mfma_shape_supported(int nwa, int ny);
 size_t mfma_lds_bytes(int precision, int nac, int nwa, int ny);
-// Writes the domain-boundary inflow into the MFMA kernels' face planes; runs
-// before the launches of every sweep.
-void launch_mfma_fill(int precision, const SweepParams& sp, hipStream_t st);
 // Several sweeps' launches in one kernel launch (one kernel variant: all
 // segments' problems share nm, nac, nwa and the precision).  probs is a
 // device array of the problems' parameters.
 void launch_mfma_multi(int precision, int nm, int nac, int nwa, int ny, unsigned grid,
                        const SweepParams* probs, const MultiLaunch& ml, hipStream_t st);
-void launch_mfma_fill_multi(int precision, unsigned grid, const SweepParams* probs,
-                            const MultiLaunch& ml, hipStream_t st);
-// Workgroups of one sweep's boundary fill.
-int mfma_fill_rows(const SweepParams& sp);
+// The same per precision (sweep_mfma_f64.hip, _f32.hip, _bf16.hip).
+void launch_mfma_f64(const LaunchSpec& ls, const SweepParams& sp, hipStream_t st);
+void launch_mfma_f32(const LaunchSpec& ls, const SweepParams& sp, hipStream_t st);
+void launch_mfma_bf16(const LaunchSpec& ls, const SweepParams& sp, hipStream_t st);
+void launch_mfma_multi_f64(int nm, int nac, int nwa, int ny, unsigned grid, size_t lds,
+                           const SweepParams* probs, const MultiLaunch& ml, hipStream_t st);
+void launch_mfma_multi_f32(int nm, int nac, int nwa, int ny, unsigned grid, size_t lds,
+                           const SweepParams* probs, const MultiLaunch& ml, hipStream_t st);
+void launch_mfma_multi_bf16(int nm, int nac, int nwa, int ny, unsigned grid, size_t lds,
+                            const SweepParams* probs, const MultiLaunch& ml, hipStream_t st);
+// The mixed forms' m2a operand copies of vi (each value divided by its
+// cell's scale K = S E U): FP32 form hi (FP32) and lo (BF16 of the rest), BF16
+// form lo = the BF16 value (hi unused).  Bytes of device memory they take
+// for a state of n values (0 in FP64).
+size_t operand_copy_bytes(int precision, size_t n);
+void launch_operand_copy(int precision, const MswDims& d, const double* vi, float* hi,
+                         unsigned short* lo, hipStream_t st);
 // Bytes of one face value of the MFMA kernels in a precision.
 size_t mfma_face_elem_bytes(int precision);
 // MFMA tile: mfma_tile_x() cells along x, ny row groups of mfma_tile_rows()
@@ -54,6 +65,9 @@ int mfma_tile_rows();
 
 // Sets the calling thread's msw_last_error() text; returns code.
 int set_error(int code, const std::string& msg);
+// Most energy groups one tile pipeline streams (the kernel keeps the groups'
+// scale exponents in one 64-bit word).
+constexpr int kMaxStreamGroups = 32;
 // Default angles per wave of the MFMA kernels for a precision and na.
 int default_mfma_nac(int precision, int na);
 // Energy groups per workgroup of the MFMA kernels: about 128 stream steps
@@ -78,8 +92,6 @@ void launch_pack(const double* afm, const double* mfa, double* at, double* mf, i
                  int na_pad, hipStream_t st);
 void launch_init_state(const MswDims& d, double* v, hipStream_t st);
 size_t norms_scratch_doubles();
-// dst <- BF16 bits of (float)src, 4 values per thread; n a multiple of 4.
-void launch_to_bf16(const double* src, unsigned short* dst, size_t n, hipStream_t st);
 void launch_norms(const double* vi, const double* vo, size_t n, double* part, double* out2,
                   hipStream_t st);
 

@@ -1,0 +1,17 @@
+// sweep_mfma_f64.hip -- the tile sweep kernels (sweep_tile.h) of the plain FP64
+// form, every workgroup shape, moment count and angles per wave; one file per
+// precision so that the three build in parallel.
+#include "sweep_tile.h"
+
+namespace msw {
+
+void launch_mfma_f64(const LaunchSpec& ls, const SweepParams& sp, hipStream_t st) {
+  launch_prec<PrecF64>(ls, sp, st);
+}
+
+void launch_mfma_multi_f64(int nm, int nac, int nwa, int ny, unsigned grid, size_t lds,
+                          const SweepParams* probs, const MultiLaunch& ml, hipStream_t st) {
+  launch_multi_prec<PrecF64>(nm, nac, nwa, ny, grid, lds, probs, ml, st);
+}
+
+}  // namespace msw

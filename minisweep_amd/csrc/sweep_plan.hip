@@ -63,7 +63,7 @@ int set_error(int code, const std::string& msg) {
 // steps per workgroup, so the 18-step fill and drain of a tile cost ~12%
 // rather than 18/(nz+18).
 int default_mfma_epw(int nz, int ne) {
-  const int e = (128 + nz / 2) / nz;
+  const int e = std::min((128 + nz / 2) / nz, kMaxStreamGroups);
   return e < 1 ? 1 : e > ne ? ne : e;
 }
 
@@ -85,7 +85,7 @@ int default_mfma_ny(int precision, int nwa, int ncell_y) {
 // pay off more: measured at 64^3).
 int launch_mfma_epw(int preferred, int ne, long long waves1, int precision) {
   const long long target = precision == MSW_PREC_FP64 ? 4096 : 2048;
-  int e = preferred < 1 ? 1 : preferred > ne ? ne : preferred;
+  int e = std::min(preferred < 1 ? 1 : preferred > ne ? ne : preferred, kMaxStreamGroups);
   while (e > 1 && waves1 * ((ne + e - 1) / e) < target) --e;
   return e;
 }
@@ -378,24 +378,6 @@ __global__ void k_pack_matrices(const double* __restrict__ afm,
 }
 
 // Manufactured state v = (1+im) S(x,y,z) E(ie) U(iu), layout [iz][ie][iy][ix][iu][im].
-// FP64 -> BF16 copy of a state (the BF16 form's input, once per sweep): each
-// value rounded to FP32, then to BF16 (nearest even), the two roundings the
-// form's contractions apply; 4 values per thread and step, n a multiple of 4.
-__global__ __launch_bounds__(256) void k_state_to_bf16(const double* __restrict__ src,
-                                                       unsigned short* __restrict__ dst,
-                                                       size_t n) {
-  for (size_t i = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) * 4; i < n;
-       i += (size_t)gridDim.x * blockDim.x * 4) {
-    const double2 a = *reinterpret_cast<const double2*>(src + i);
-    const double2 b = *reinterpret_cast<const double2*>(src + i + 2);
-    const unsigned short h0 = __builtin_bit_cast(unsigned short, (__bf16)(float)a.x);
-    const unsigned short h1 = __builtin_bit_cast(unsigned short, (__bf16)(float)a.y);
-    const unsigned short h2 = __builtin_bit_cast(unsigned short, (__bf16)(float)b.x);
-    const unsigned short h3 = __builtin_bit_cast(unsigned short, (__bf16)(float)b.y);
-    *reinterpret_cast<uint2*>(dst + i) = uint2{h0 | (unsigned)h1 << 16, h2 | (unsigned)h3 << 16};
-  }
-}
-
 __global__ void k_init_state(double* __restrict__ v, int nx, int ny, int nz,
                              int ne, int nm, int nu, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
@@ -556,11 +538,6 @@ void launch_init_state(const MswDims& d, double* v, hipStream_t st) {
 
 size_t norms_scratch_doubles() { return 2 * (size_t)kNormBlocks; }
 
-void launch_to_bf16(const double* src, unsigned short* dst, size_t n, hipStream_t st) {
-  const unsigned blocks = (unsigned)std::min<size_t>((n / 4 + 255) / 256, 8192);
-  hipLaunchKernelGGL(k_state_to_bf16, dim3(blocks), dim3(256), 0, st, src, dst, n);
-}
-
 void launch_norms(const double* vi, const double* vo, size_t n, double* part, double* out2,
                   hipStream_t st) {
   hipLaunchKernelGGL(k_norms_partial, dim3(kNormBlocks), dim3(kNormThreads), 0, st, vi, vo, n,
@@ -585,7 +562,10 @@ struct MswPlan {
   void* fyz;   // face planes (element type of the precision)
   void* fxz;
   float* vo32;  // mixed forms: FP32 running vo sum over the octants
-  unsigned short* vib;  // BF16 form: BF16 copy of vi
+  // mixed forms: the m2a operand copies of vi (FP32 form: vhi and vlo, BF16
+  // form: vlo as the BF16 copy), made at the start of every sweep
+  float* vhi;
+  unsigned short* vlo;
   double* norm_part;
   TileItem* items;
   std::vector<int> launch_off, launch_cnt;
@@ -723,6 +703,8 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
   if (precision != MSW_PREC_FP64 && opt->engine != MSW_ENGINE_MFMA)
     return set_err(MSW_ERR_UNSUPPORTED,
                    "mixed precision runs on the MFMA engine only (precision %d)", precision);
+  if (precision != MSW_PREC_FP64 && msw_size_state(d) / d->nm >= (1ull << 32))
+    return set_err(MSW_ERR_UNSUPPORTED, "mixed precision: state of 2^32 or more cells x unknowns");
   if (d->nm != 4 && d->nm != 8 && d->nm != 16)
     return set_err(MSW_ERR_UNSUPPORTED, "nm must be 4, 8 or 16 (got %d)", d->nm);
   if (opt->engine != MSW_ENGINE_VALU && opt->engine != MSW_ENGINE_MFMA)
@@ -820,7 +802,7 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
   for (auto& l : launches) {
     const long long w1 = (long long)l.size() * d->nu * p->nwa * p->ny;
     p->launch_epw.push_back(opt->groups_per_workgroup > 0
-                                ? std::min(opt->groups_per_workgroup, d->ne)
+                                ? std::min({opt->groups_per_workgroup, d->ne, kMaxStreamGroups})
                                 : launch_mfma_epw(default_mfma_epw(p->zb, d->ne), d->ne, w1,
                                                   precision));
   }
@@ -853,15 +835,12 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
     p->bytes += sizeof(double) * nfz;
   }
   if (e == hipSuccess && p->engine == MSW_ENGINE_MFMA && precision != MSW_PREC_FP64) {
-    e = hipMalloc(&p->vo32, sizeof(float) * msw_size_state(d));
-    p->bytes += sizeof(float) * msw_size_state(d);
-    // the BF16 form reads a BF16 copy of vi (2 bytes a value, measured faster
-    // at 64^3); the FP32 form reads vi itself, an FP32 copy would cost more
-    // than it saves
-    if (e == hipSuccess && precision == MSW_PREC_MIXED_BF16) {
-      e = hipMalloc(&p->vib, sizeof(unsigned short) * msw_size_state(d));
-      p->bytes += sizeof(unsigned short) * msw_size_state(d);
-    }
+    const size_t n = msw_size_state(d);
+    e = hipMalloc(&p->vo32, sizeof(float) * n);
+    p->bytes += sizeof(float) * n + operand_copy_bytes(precision, n);
+    if (e == hipSuccess) e = hipMalloc(&p->vlo, sizeof(unsigned short) * n);
+    if (e == hipSuccess && precision == MSW_PREC_MIXED_FP32)
+      e = hipMalloc(&p->vhi, sizeof(float) * n);
   }
   if (e == hipSuccess) e = hipMalloc(&p->norm_part, sizeof(double) * 2 * kNormBlocks);
   if (e == hipSuccess) e = hipMalloc(&p->items, sizeof(TileItem) * flat.size());
@@ -925,7 +904,8 @@ int msw_plan_destroy(MswPlan* p) {
   if (p->fyz) (void)hipFree(p->fyz);
   if (p->fxz) (void)hipFree(p->fxz);
   if (p->vo32) (void)hipFree(p->vo32);
-  if (p->vib) (void)hipFree(p->vib);
+  if (p->vhi) (void)hipFree(p->vhi);
+  if (p->vlo) (void)hipFree(p->vlo);
   if (p->norm_part) (void)hipFree(p->norm_part);
   if (p->items) (void)hipFree(p->items);
   if (p->fz) (void)hipFree(p->fz);
@@ -947,8 +927,8 @@ int msw_plan_destroy(MswPlan* p) {
 
 // The launches of one sweep on device-resident vi, vo.  A plan with z blocks
 // runs two passes (octants 0-3 upward in z, then 4-7 downward), each over the
-// blocks in its direction: a block's boundary fill, then the pass's launches
-// on that block's planes.
+// blocks in its direction: the pass's launches on each block's planes.  The
+// mixed forms first make their operand copies of vi.
 static int sweep_device(MswPlan* p, const double* vi, double* vo, hipStream_t st) {
   const MswDims& d = p->d;
   SweepParams sp;
@@ -959,7 +939,9 @@ static int sweep_device(MswPlan* p, const double* vi, double* vo, hipStream_t st
   sp.fyz = p->fyz;
   sp.fxz = p->fxz;
   sp.vo32 = p->vo32;
-  sp.vib = p->vib;
+  sp.vhi = p->vhi;
+  sp.vlo = p->vlo;
+  sp.vib = p->vlo;
   sp.items = p->items;
   sp.nx = d.ncell_x;
   sp.ny = d.ncell_y;
@@ -971,8 +953,7 @@ static int sweep_device(MswPlan* p, const double* vi, double* vo, hipStream_t st
   sp.ntx = p->ntx;
   sp.nty = p->nty;
   if (p->nblk == 1) {
-    if (p->vib) launch_to_bf16(vi, p->vib, msw_size_state(&d), st);
-    if (p->engine == MSW_ENGINE_MFMA) launch_mfma_fill(p->precision, sp, st);
+    if (p->vlo) launch_operand_copy(p->precision, d, vi, p->vhi, p->vlo, st);
     return launch_range(p, sp, st, 0, p->launch_off.size());
   }
   const size_t slab = msw_size_state(&d) / p->nblk;
@@ -988,7 +969,6 @@ static int sweep_device(MswPlan* p, const double* vi, double* vo, hipStream_t st
       b.fz = p->fz;
       b.z_in = kk > 0;
       b.z_out = kk < p->nblk - 1;
-      launch_mfma_fill(p->precision, b, st);
       const int rc = launch_range(p, b, st, pass == 0 ? 0 : p->pass_split,
                                   pass == 0 ? p->pass_split : p->launch_off.size());
       if (rc) return rc;
@@ -1098,6 +1078,8 @@ int msw_plan_sweep_host(MswPlan* p, const double* vi_host, double* vo_host,
       b.fyz = p->fyz;
       b.fxz = p->fxz;
       b.vo32 = nullptr;
+      b.vhi = nullptr;
+      b.vlo = nullptr;
       b.vib = nullptr;
       b.items = p->items;
       b.nx = d.ncell_x;
@@ -1114,7 +1096,6 @@ int msw_plan_sweep_host(MswPlan* p, const double* vi_host, double* vo_host,
       b.fz = p->fz;
       b.z_in = s.kk > 0;
       b.z_out = s.kk < p->nblk - 1;
-      launch_mfma_fill(p->precision, b, st);
       rc = launch_range(p, b, st, s.pass == 0 ? 0 : p->pass_split,
                         s.pass == 0 ? p->pass_split : p->launch_off.size());
     }

@@ -9,10 +9,13 @@ Stated error bars, relative to the FP64 result:
     (state, matrices, faces) is a small integer times a power of two that
     BF16/FP32 hold exactly, and all partial sums stay exact in FP32.
   * perturbed state, FP32 form: normsq within 1e-5 of the FP64 oracle (the
-    stated target) and vo normwise (max |d| / max |ref|) within 1e-4.
-    The floor is FP32 accumulation: the rows of A and M are second
-    differences weighted by 21..55 that cancel the smooth part of the input,
-    so rounding at 2^-24 of the summands shows up ~300x larger in the result.
+    stated target) and vo normwise (max |d| / max |ref|) within 1e-4.  The
+    rows of A and M are second differences weighted by 21..55 that cancel
+    the smooth part of the input, so roundings show up ~300x larger in the
+    result.  Rounding vi itself to FP32 was the largest term (normsq errors
+    up to 2.4e-5 on these shapes); the form carries vi's rounding residual in
+    BF16 through a second m2a MFMA, which leaves the FP32 accumulation as the
+    floor (a CPU emulation over seeds: 0.9e-6 .. 2.4e-6).
   * perturbed state, BF16 form: no accuracy claim.  Rounding the state and
     the solved angular values to BF16 (2^-9) before the same cancellation
     leaves O(1) normwise error (measured 0.5 .. 1.2); the test only checks
@@ -82,13 +85,7 @@ def test_mixed_manufactured(shape, prec, cuda):
 
 @pytest.mark.parametrize("prec", PRECS, ids=PREC_IDS)
 @pytest.mark.parametrize("shape", SHAPES, ids=[str(s) for s in SHAPES])
-def test_mixed_perturbed_vs_oracle(shape, prec, cuda, request):
-    if prec == msw.PREC_MIXED_FP32 and shape == (17, 3, 5, 1, 8, 40, 2):
-        # measured normsq 2.36e-5: rounding vi to FP32 on load is the largest
-        # error term on this input (a CPU emulation of the FP32 form over
-        # seeds gives 1.5e-6 .. 2.3e-5, and 0.9e-6 .. 2.4e-6 with vi kept
-        # exact); the form does not yet carry vi's rounding residual
-        request.applymarker(pytest.mark.xfail(reason="FP32 rounding of vi", strict=False))
+def test_mixed_perturbed_vs_oracle(shape, prec, cuda):
     import torch
     print(f"case {shape} prec {prec}", flush=True)
     d = msw.dims(*shape)
