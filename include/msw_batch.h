@@ -65,7 +65,9 @@ int msw_batch_run(MswBatch* batch, void* stream);
  * turn, then the tile height in row groups, the energy groups streamed per
  * workgroup, the classes' angles per wave once more, the schedule heuristic
  * and staggering) over reps runs and keep the fastest.  Runs the batch many
- * times; the problems' vo are overwritten. */
+ * times; the problems' vo are overwritten.  Returns MSW_ERR_UNSUPPORTED unless
+ * the environment sets MSW_BATCH_AUTOTUNE=1 (a GPU run of it ended with a host
+ * fault of unknown cause); the batch then keeps its default configuration. */
 int msw_batch_autotune(MswBatch* batch, void* stream, int reps);
 
 /* Angles per wave used for problems with this (nm, na) (0 = default). */

@@ -3,9 +3,11 @@ sequential loop of plain FP64 sweeps of the same problems.
 
 Problems: 256 drawn from seed 0 (minisweep_amd.workloads.uneven_problems),
 manufactured input state (Quantities_init_state), one sweep each.  The batch
-runs the FP32-MFMA mixed form by default, as one hipGraph replay, after the
-autotuner has picked angles per wave per (nm, na) class and the schedule
-heuristic; the FP64 batch is timed too, to separate batching from precision.
+runs the FP32-MFMA mixed form by default, as one hipGraph replay, in its
+default configuration (--autotune, with MSW_BATCH_AUTOTUNE=1 in the
+environment, lets the autotuner pick angles per wave per (nm, na) class and
+the schedule heuristic first); the FP64 batch is timed too, to separate
+batching from precision.
 Every result is checked: normsq equals the closed form and normsqdiff is 0.
 
     python -m minisweep_amd.batch_bench [--nprob 256] [--seed 0] [--reps 5]
@@ -27,7 +29,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--precision", choices=["fp32", "bf16", "fp64"], default="fp32")
-    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="autotune the batch first (needs MSW_BATCH_AUTOTUNE=1)")
     args = ap.parse_args()
 
     import torch
@@ -99,7 +102,7 @@ def main():
     bad_b64 = check()
     b64.close()
 
-    bm, t_tune = batch_of(prec, not args.no_autotune)
+    bm, t_tune = batch_of(prec, args.autotune)
     t_bm = time_runs(bm.run, args.reps)
     bad_bm = check()
     desc = bm.describe()

@@ -36,16 +36,26 @@ size_t mfma_lds_bytes(int precision, int nac, int nwa, int ny);
 // device array of the problems' parameters.
 void launch_mfma_multi(int precision, int nm, int nac, int nwa, int ny, unsigned grid,
                        const SweepParams* probs, const MultiLaunch& ml, hipStream_t st);
-// The same per precision (sweep_mfma_f64.hip, _f32.hip, _bf16.hip).
-void launch_mfma_f64(const LaunchSpec& ls, const SweepParams& sp, hipStream_t st);
-void launch_mfma_f32(const LaunchSpec& ls, const SweepParams& sp, hipStream_t st);
-void launch_mfma_bf16(const LaunchSpec& ls, const SweepParams& sp, hipStream_t st);
-void launch_mfma_multi_f64(int nm, int nac, int nwa, int ny, unsigned grid, size_t lds,
-                           const SweepParams* probs, const MultiLaunch& ml, hipStream_t st);
-void launch_mfma_multi_f32(int nm, int nac, int nwa, int ny, unsigned grid, size_t lds,
-                           const SweepParams* probs, const MultiLaunch& ml, hipStream_t st);
-void launch_mfma_multi_bf16(int nm, int nac, int nwa, int ny, unsigned grid, size_t lds,
-                            const SweepParams* probs, const MultiLaunch& ml, hipStream_t st);
+// The same per (precision, nm): sweep_mfma_inst.hip compiled once for each.
+#define MSW_LAUNCH_NAME_(P, NM) launch_mfma_##P##_nm##NM
+#define MSW_LAUNCH_NAME(P, NM) MSW_LAUNCH_NAME_(P, NM)
+#define MSW_LAUNCH_MULTI_NAME_(P, NM) launch_mfma_multi_##P##_nm##NM
+#define MSW_LAUNCH_MULTI_NAME(P, NM) MSW_LAUNCH_MULTI_NAME_(P, NM)
+#define MSW_DECLARE_LAUNCH(P, NM)                                                          \
+  void MSW_LAUNCH_NAME(P, NM)(const LaunchSpec& ls, const SweepParams& sp, hipStream_t st); \
+  void MSW_LAUNCH_MULTI_NAME(P, NM)(int nac, int nwa, int ny, unsigned grid, size_t lds,   \
+                                    const SweepParams* probs, const MultiLaunch& ml,       \
+                                    hipStream_t st);
+MSW_DECLARE_LAUNCH(F64, 4)
+MSW_DECLARE_LAUNCH(F64, 8)
+MSW_DECLARE_LAUNCH(F64, 16)
+MSW_DECLARE_LAUNCH(F32, 4)
+MSW_DECLARE_LAUNCH(F32, 8)
+MSW_DECLARE_LAUNCH(F32, 16)
+MSW_DECLARE_LAUNCH(BF16, 4)
+MSW_DECLARE_LAUNCH(BF16, 8)
+MSW_DECLARE_LAUNCH(BF16, 16)
+#undef MSW_DECLARE_LAUNCH
 // The mixed forms' m2a operand copies of vi (each value divided by its
 // cell's scale K = S E U): FP32 form hi (FP32) and lo (BF16 of the rest), BF16
 // form lo = the BF16 value (hi unused).  Bytes of device memory they take

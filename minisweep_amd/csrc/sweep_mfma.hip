@@ -1,7 +1,7 @@
 // sweep_mfma.hip -- dispatch of the tile sweep kernels (sweep_tile.h) by
 // precision, workgroup shapes, and the mixed forms' operand copies of vi.
-// The kernels themselves are instantiated one precision per file
-// (sweep_mfma_f64.hip, sweep_mfma_f32.hip, sweep_mfma_bf16.hip).
+// The kernels themselves are instantiated by sweep_mfma_inst.hip, compiled once
+// per (precision, nm).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -13,9 +13,6 @@
 namespace msw {
 
 namespace {
-constexpr int kTileX = 16;    // tile cells along x: one 16-lane row
-constexpr int kTileRows = 4;  // cell rows per row group
-
 // The mixed forms' m2a operand, made once per sweep: each value of vi
 // divided by its cell's scale K = 2^((x + 3y + 7z + 2) & 3) E(group) U(unknown)
 // (a power of two: exact), then
@@ -64,30 +61,36 @@ __global__ __launch_bounds__(256) void k_operand_copy(const double* __restrict__
 }
 }  // namespace
 
-bool mfma_shape_supported(int nwa, int ny) {
-  return (ny == 1 && nwa >= 1 && nwa <= 4) || (ny == 2 && (nwa == 1 || nwa == 2)) ||
-         (ny == 4 && nwa == 1);
-}
+#define MSW_BY_NM(P, call, ...)                  \
+  switch (nm) {                                  \
+    case 4: call(P, 4)(__VA_ARGS__); break;      \
+    case 8: call(P, 8)(__VA_ARGS__); break;      \
+    default: call(P, 16)(__VA_ARGS__); break;    \
+  }
 
 void launch_mfma(int precision, const LaunchSpec& ls, const SweepParams& sp, hipStream_t st) {
-  if (precision == MSW_PREC_MIXED_FP32)
-    launch_mfma_f32(ls, sp, st);
-  else if (precision == MSW_PREC_MIXED_BF16)
-    launch_mfma_bf16(ls, sp, st);
-  else
-    launch_mfma_f64(ls, sp, st);
+  const int nm = ls.nm;
+  if (precision == MSW_PREC_MIXED_FP32) {
+    MSW_BY_NM(F32, MSW_LAUNCH_NAME, ls, sp, st)
+  } else if (precision == MSW_PREC_MIXED_BF16) {
+    MSW_BY_NM(BF16, MSW_LAUNCH_NAME, ls, sp, st)
+  } else {
+    MSW_BY_NM(F64, MSW_LAUNCH_NAME, ls, sp, st)
+  }
 }
 
 void launch_mfma_multi(int precision, int nm, int nac, int nwa, int ny, unsigned grid,
                        const SweepParams* probs, const MultiLaunch& ml, hipStream_t st) {
   const size_t lds = mfma_lds_bytes(precision, nac, nwa, ny);
-  if (precision == MSW_PREC_MIXED_FP32)
-    launch_mfma_multi_f32(nm, nac, nwa, ny, grid, lds, probs, ml, st);
-  else if (precision == MSW_PREC_MIXED_BF16)
-    launch_mfma_multi_bf16(nm, nac, nwa, ny, grid, lds, probs, ml, st);
-  else
-    launch_mfma_multi_f64(nm, nac, nwa, ny, grid, lds, probs, ml, st);
+  if (precision == MSW_PREC_MIXED_FP32) {
+    MSW_BY_NM(F32, MSW_LAUNCH_MULTI_NAME, nac, nwa, ny, grid, lds, probs, ml, st)
+  } else if (precision == MSW_PREC_MIXED_BF16) {
+    MSW_BY_NM(BF16, MSW_LAUNCH_MULTI_NAME, nac, nwa, ny, grid, lds, probs, ml, st)
+  } else {
+    MSW_BY_NM(F64, MSW_LAUNCH_MULTI_NAME, nac, nwa, ny, grid, lds, probs, ml, st)
+  }
 }
+#undef MSW_BY_NM
 
 size_t operand_copy_bytes(int precision, size_t n) {
   return precision == MSW_PREC_MIXED_FP32   ? n * (sizeof(float) + sizeof(unsigned short))
@@ -105,24 +108,5 @@ void launch_operand_copy(int precision, const MswDims& d, const double* vi, floa
                      d.ncell_y, d.ne, d.nm, d.nu);
 }
 
-size_t mfma_face_elem_bytes(int precision) {
-  return precision == MSW_PREC_FP64 ? sizeof(double) : sizeof(float);
-}
-
-size_t mfma_lds_bytes(int precision, int nac, int nwa, int ny) {
-  const size_t t = mfma_face_elem_bytes(precision);  // accumulator = face type
-  const size_t nab = nac / 16;
-  const size_t red = nwa > 1 ? 2 * (size_t)ny * kTileRows * nwa * 64 * 4 * t : 0;
-  const size_t ring = ny > 1 ? 2 * (size_t)(ny - 1) * nwa * nab * 64 * 4 * t : 0;
-  return red + ring;
-}
-
-bool mfma_face_offsets_fit(const MswDims& d, int na_pad) {
-  const long long ext = d.ncell_x > d.ncell_y ? d.ncell_x : d.ncell_y;
-  return ext * d.ne * d.ncell_z * na_pad < (1ll << 31);
-}
-
-int mfma_tile_x() { return kTileX; }
-int mfma_tile_rows() { return kTileRows; }
 
 }  // namespace msw

@@ -41,8 +41,6 @@
 
 namespace msw {
 
-static thread_local std::string g_err;
-
 static int set_err(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 static int set_err(int code, const char* fmt, ...) {
   char buf[512];
@@ -50,57 +48,10 @@ static int set_err(int code, const char* fmt, ...) {
   va_start(ap, fmt);
   vsnprintf(buf, sizeof(buf), fmt, ap);
   va_end(ap);
-  g_err = buf;
-  return code;
+  return set_error(code, buf);
 }
 
-int set_error(int code, const std::string& msg) {
-  g_err = msg;
-  return code;
-}
 
-// Energy groups streamed through one MFMA tile pipeline: about 128 stream
-// steps per workgroup, so the 18-step fill and drain of a tile cost ~12%
-// rather than 18/(nz+18).
-int default_mfma_epw(int nz, int ne) {
-  const int e = std::min((128 + nz / 2) / nz, kMaxStreamGroups);
-  return e < 1 ? 1 : e > ne ? ne : e;
-}
-
-// Row groups of 4 rows per MFMA tile.  Measured at 64^3, na=64: FP64 (32
-// angles per wave, two waves per tile) is fastest with one row group (the
-// per-step barrier of four waves costs more than the y-face traffic it
-// saves); the mixed forms (64 angles per wave) with two.  Never more rows
-// than the grid has.
-int default_mfma_ny(int precision, int nwa, int ncell_y) {
-  const int want = precision == MSW_PREC_FP64 ? 1 : 2;
-  for (int ny = want; ny > 1; ny /= 2)
-    if (mfma_shape_supported(nwa, ny) && 4 * ny <= ((ncell_y + 3) / 4) * 4) return ny;
-  return 1;
-}
-
-// Energy groups per workgroup for one launch of waves1 waves per group: the
-// preferred count, lowered while the launch would have fewer waves than the
-// target (16 per SIMD for FP64, 8 for the mixed forms, whose longer streams
-// pay off more: measured at 64^3).
-int launch_mfma_epw(int preferred, int ne, long long waves1, int precision) {
-  const long long target = precision == MSW_PREC_FP64 ? 4096 : 2048;
-  int e = std::min(preferred < 1 ? 1 : preferred > ne ? ne : preferred, kMaxStreamGroups);
-  while (e > 1 && waves1 * ((ne + e - 1) / e) < target) --e;
-  return e;
-}
-
-// Default angles per wave of the MFMA kernels: measured at 64^3, na=64, 32
-// per wave is fastest in FP64 (16 loses more to the cross-wave vo reduction
-// than it gains in occupancy) and 64 in the mixed forms; never more than na
-// rounded up to a 16-angle block, and at most 4 waves per tile.
-int default_mfma_nac(int precision, int na) {
-  const int cap = na <= 16 ? 16 : na <= 32 ? 32 : 64;
-  int nac = precision == MSW_PREC_FP64 ? 32 : 64;
-  nac = nac < cap ? nac : cap;
-  while ((na + nac - 1) / nac > 4 && nac < 64) nac *= 2;
-  return nac;
-}
 
 #define HIP_TRY(call)                                                        \
   do {                                                                       \
@@ -450,77 +401,6 @@ __global__ __launch_bounds__(kNormThreads) void k_norms_final(
   }
 }
 
-// ---------------------------------------------------------------------------
-// Host-side schedule: list scheduling of (octant, tile) visits.
-//
-// Octant o's tile (i, j) in its own frame needs (i-1, j) and (i, j-1) done.
-// A launch takes every ready visit whose physical tile is not yet taken in
-// that launch.  Octant priority: kind 0, most remaining visits first; kind 1,
-// the reference's fixed octant order {0,4,2,6,3,7,1,5}
-// (src/3_sweeper/stepscheduler_kba.c:155).  Returns the launches in order;
-// marks the first and the last visit of every physical tile.
-std::vector<std::vector<TileItem>> build_schedule(int ntx, int nty, int kind, unsigned oct_mask,
-                                                  int touched0) {
-  static const int kRefOrder[kNoct] = {0, 4, 2, 6, 3, 7, 1, 5};
-  std::vector<std::vector<TileItem>> launches;
-  std::vector<std::vector<int>> done(kNoct, std::vector<int>(ntx * nty, 0));
-  std::vector<int> remaining(kNoct, 0);
-  int left = 0;
-  for (int o = 0; o < kNoct; ++o)
-    if (oct_mask >> o & 1) {
-      remaining[o] = ntx * nty;
-      left += ntx * nty;
-    }
-  // visits so far per physical tile (touched0 from earlier passes)
-  std::vector<int> touched(ntx * nty, touched0);
-  while (left > 0) {
-    std::vector<int> order;
-    for (int o = 0; o < kNoct; ++o) {
-      const int oc = kind == 1 ? kRefOrder[o] : o;
-      if (oct_mask >> oc & 1) order.push_back(oc);
-    }
-    if (kind != 1)
-      std::stable_sort(order.begin(), order.end(),
-                       [&](int a, int b) { return remaining[a] > remaining[b]; });
-    std::vector<int> used(ntx * nty, 0);
-    std::vector<TileItem> items;
-    for (int o : order) {
-      // ready visits, nearest the upwind corner first
-      for (int d = 0; d <= ntx + nty - 2; ++d)
-        for (int i = 0; i < ntx; ++i) {
-          const int j = d - i;
-          if (j < 0 || j >= nty) continue;
-          if (done[o][i + ntx * j]) continue;
-          if (i > 0 && done[o][(i - 1) + ntx * j] != 1) continue;
-          if (j > 0 && done[o][i + ntx * (j - 1)] != 1) continue;
-          const int gx = (o & 1) ? ntx - 1 - i : i;
-          const int gy = (o & 2) ? nty - 1 - j : j;
-          const int g = gx + ntx * gy;
-          if (used[g]) continue;
-          used[g] = 1;
-          TileItem t;
-          t.octant = o;
-          t.tx = i;
-          t.ty = j;
-          t.flags = (touched[g] == 0 ? kVisitFirst : 0) | (touched[g] == kNoct - 1 ? kVisitLast : 0);
-          items.push_back(t);
-        }
-    }
-    for (const TileItem& t : items) {
-      done[t.octant][t.tx + ntx * t.ty] = 2;  // finished in this launch
-      const int gx = (t.octant & 1) ? ntx - 1 - t.tx : t.tx;
-      const int gy = (t.octant & 2) ? nty - 1 - t.ty : t.ty;
-      touched[gx + ntx * gy]++;
-      remaining[t.octant]--;
-      left--;
-    }
-    for (int o = 0; o < kNoct; ++o)
-      for (auto& v : done[o])
-        if (v == 2) v = 1;
-    launches.push_back(items);
-  }
-  return launches;
-}
 
 void launch_pack(const double* afm, const double* mfa, double* at, double* mf, int nm, int na,
                  int na_pad, hipStream_t st) {
@@ -642,7 +522,6 @@ static int launch_range(MswPlan* p, const SweepParams& sp, hipStream_t st, size_
 
 extern "C" {
 
-const char* msw_last_error(void) { return g_err.c_str(); }
 
 size_t msw_size_state(const MswDims* d) {
   return (size_t)d->ncell_x * d->ncell_y * d->ncell_z * d->ne * d->nm * d->nu;
@@ -1202,55 +1081,5 @@ int msw_init_state(const MswDims* d, double* v, void* stream) {
   return MSW_OK;
 }
 
-int msw_init_matrices_host(int nm, int na, double* A, double* M) {
-  if (nm <= 0 || na <= 0 || !A || !M)
-    return set_err(MSW_ERR_INVALID, "bad matrix init arguments");
-  // Quantities_init_am_matrices_ (src/3_sweeper/quantities_testing_c.h:40-201):
-  // integer rows that send an affine vector in moments to an affine vector in
-  // angles and back, plus rows that annihilate affine vectors.
-  const size_t n = (size_t)kNoct * nm * na;
-  std::fill(A, A + n, 0.0);
-  std::fill(M, M + n, 0.0);
-  auto afm = [&](int o, int m, int a) -> double& { return A[a + (size_t)na * (m + (size_t)nm * o)]; };
-  auto mfa = [&](int o, int m, int a) -> double& { return M[m + (size_t)nm * (a + (size_t)na * o)]; };
-  for (int o = 0; o < kNoct; ++o) {
-    for (int i = 0; i < na; ++i) {
-      const int q = (i + 1) / nm, r = (i + 1) % nm;
-      afm(o, nm - 1, i) += q;
-      if (r) {
-        afm(o, 0, i) -= 1.0;
-        afm(o, r, i) += 1.0;
-      }
-    }
-    for (int m = 0; m < nm - 2; ++m)
-      for (int a = 0; a < na; ++a) {
-        const int rv = 21 + (m + nm * a) % 17;
-        afm(o, m, a) += -1.0 * rv;
-        afm(o, m + 1, a) += 2.0 * rv;
-        afm(o, m + 2, a) += -1.0 * rv;
-      }
-    for (int i = 0; i < nm; ++i) {
-      const int q = (i + 1) / na, r = (i + 1) % na;
-      mfa(o, i, na - 1) += q;
-      if (r) {
-        mfa(o, i, 0) -= 1.0;
-        mfa(o, i, r) += 1.0;
-      }
-    }
-    for (int m = 0; m < nm; ++m)
-      for (int a = 0; a < na - 2; ++a) {
-        const int rv = 37 + (m + nm * a) % 19;
-        mfa(o, m, a) += -1.0 * rv;
-        mfa(o, m, a + 1) += 2.0 * rv;
-        mfa(o, m, a + 2) += -1.0 * rv;
-      }
-    for (int m = 0; m < nm; ++m)
-      for (int a = 0; a < na; ++a) {
-        mfa(o, m, a) /= kNoct;
-        mfa(o, m, a) /= (double)(1 << (a & 7));
-      }
-  }
-  return MSW_OK;
-}
 
 }  // extern "C"

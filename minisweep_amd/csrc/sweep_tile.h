@@ -945,15 +945,6 @@ static void launch_nm(const LaunchSpec& ls, const SweepParams& sp, hipStream_t s
     launch_nac<PR, NM, 64>(ls, sp, st);
 }
 
-template <class PR>
-static void launch_prec(const LaunchSpec& ls, const SweepParams& sp, hipStream_t st) {
-  switch (ls.nm) {
-    case 4: launch_nm<PR, 4>(ls, sp, st); break;
-    case 8: launch_nm<PR, 8>(ls, sp, st); break;
-    default: launch_nm<PR, 16>(ls, sp, st); break;
-  }
-}
-
 template <class PR, int NM, int NAC>
 static void launch_multi_nac(int nwa, int ny, unsigned grid, size_t lds, const SweepParams* probs,
                              const MultiLaunch& ml, hipStream_t st) {
@@ -977,16 +968,6 @@ static void launch_multi_nm(int nac, int nwa, int ny, unsigned grid, size_t lds,
     launch_multi_nac<PR, NM, 32>(nwa, ny, grid, lds, probs, ml, st);
   else
     launch_multi_nac<PR, NM, 64>(nwa, ny, grid, lds, probs, ml, st);
-}
-
-template <class PR>
-static void launch_multi_prec(int nm, int nac, int nwa, int ny, unsigned grid, size_t lds,
-                              const SweepParams* probs, const MultiLaunch& ml, hipStream_t st) {
-  switch (nm) {
-    case 4: launch_multi_nm<PR, 4>(nac, nwa, ny, grid, lds, probs, ml, st); break;
-    case 8: launch_multi_nm<PR, 8>(nac, nwa, ny, grid, lds, probs, ml, st); break;
-    default: launch_multi_nm<PR, 16>(nac, nwa, ny, grid, lds, probs, ml, st); break;
-  }
 }
 
 }  // namespace msw

@@ -13,10 +13,11 @@ Latency = completion - submission on the server clock; throughput = requests
 submitted at once) is measured for both.  Every result is checked against the
 closed-form normsq.
 
-Before serving, the served form is tuned once: a batch of --tune-problems
-problems from the same generator (seed 0, the batched workload) is autotuned
-and the server takes its kernel configuration (msw_server_tune_from_batch).
---no-tune serves with the plan defaults.
+By default the server runs the plan defaults.  With --tune (and
+MSW_BATCH_AUTOTUNE=1 in the environment) the served form is tuned once first:
+a batch of --tune-problems problems from the same generator (seed 0, the
+batched workload) is autotuned and the server takes its kernel configuration
+(msw_server_tune_from_batch).
 
     python -m minisweep_amd.serve_bench [--rates 50,200,800] [--requests 10000]
 
@@ -126,7 +127,8 @@ def main():
     ap.add_argument("--max-active", type=int, default=256)
     ap.add_argument("--precision", choices=["fp32", "bf16", "fp64"], default="fp32")
     ap.add_argument("--skip-baseline", action="store_true")
-    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune on the batched workload first (needs MSW_BATCH_AUTOTUNE=1)")
     ap.add_argument("--tune-problems", type=int, default=256)
     args = ap.parse_args()
 
@@ -149,7 +151,7 @@ def main():
         out.append(rec)
 
     tune, bufs, tuned_desc = None, None, None
-    if not args.no_tune:
+    if args.tune:
         progress(f"autotuning the served form on {args.tune_problems} problems")
         tune, bufs = tuned_batch(msw, prec, args.tune_problems)
         tuned_desc = tune.describe()
