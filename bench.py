@@ -16,12 +16,16 @@ Prints ONE JSON line.  `value` is the FP64 sweep's GFLOP/s.  Also in it:
   roofline     the dominant kernel (k_sweep_mfma) against the FP64 peak, from
                per-launch HIP events recorded on the sweep's stream during the
                timed steps; HBM traffic from the committed rocprofv3 summary.
-  cpu_baseline the reference's own CPU sweep (its serial KBA build, compiled
-               from its sources into oracle/_ref/ by oracle/Makefile) timed on
-               this host's core over one energy group of the same shape, one
-               sweep; beside it ("port") the CPU oracle (oracle/, the
-               restatement of the reference algorithm) with OpenMP over a few
-               energy groups.  Without oracle/_ref the oracle alone.
+  cpu_baseline the reference's own CPU sweep, its OpenMP-threads build
+               (compiled from its sources into oracle/_ref/ by oracle/Makefile),
+               on this host's cores (16 at most), over as many energy groups
+               of the same shape, one sweep; beside it its serial build on one
+               core and one group ("reference_serial"), and the CPU oracle
+               (oracle/, the restatement of the reference algorithm) with
+               OpenMP on the same cores ("port").  Without oracle/_ref the
+               oracle alone, with the reason in "reference_missing".
+  mixed_fp64_last  the mixed FP32 form with a final FP64 sweep: of four
+               iterations the first three in FP32, the last in FP64.
 """
 from __future__ import annotations
 
@@ -88,32 +92,45 @@ def cpu_baseline(ncx, ncy, ncz, nm, na, nu, threads):
     }
 
 
-def cpu_baseline_reference(ncx, ncy, ncz, nm, na, nu):
-    """The reference's serial sweep driver (oracle/_ref/nm<NM>/sweep) on one
-    energy group of the shape: its own result line gives time and GF/s."""
+def cpu_baseline_reference(ncx, ncy, ncz, nm, na, nu, threads=1):
+    """The reference's own sweep driver on this host, one sweep: its serial
+    KBA build (oracle/_ref/nm<NM>/sweep) on one energy group, or with
+    threads > 1 its OpenMP-threads build (oracle/_ref/nm16_omp/sweep,
+    --nthread_e threads) on `threads` groups.  Both are compiled from the
+    reference's sources by oracle/Makefile (make ref); its own result line
+    gives time and GF/s.  Returns (result, None) or (None, why)."""
     import re
     import subprocess
-    exe = os.path.join(REPO, "oracle", "_ref", f"nm{nm}", "sweep")
-    if nu != 4 or not os.access(exe, os.X_OK):  # the reference fixes NU = 4
-        return None
+    if threads > 1:
+        exe = os.path.join(REPO, "oracle", "_ref", f"nm{nm}_omp", "sweep")
+        ne, extra, build = threads, ["--nthread_e", str(threads)], "OpenMP-threads"
+    else:
+        exe = os.path.join(REPO, "oracle", "_ref", f"nm{nm}", "sweep")
+        ne, extra, build = 1, [], "serial KBA"
+    if nu != 4:  # the reference fixes NU = 4
+        return None, "the reference is built for nu = 4"
+    if not os.access(exe, os.X_OK):
+        return None, f"{os.path.relpath(exe, REPO)} is not built (oracle/Makefile ref)"
     cmd = [exe, "--ncell_x", str(ncx), "--ncell_y", str(ncy), "--ncell_z", str(ncz),
-           "--ne", "1", "--na", str(na), "--niterations", "1"]
+           "--ne", str(ne), "--na", str(na), "--niterations", "1"] + extra
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     try:
-        out = subprocess.run(cmd, capture_output=True, text=True, timeout=300).stdout
-    except (OSError, subprocess.TimeoutExpired):
-        return None
+        out = subprocess.run(cmd, capture_output=True, text=True, timeout=300, env=env).stdout
+    except (OSError, subprocess.TimeoutExpired) as e:
+        return None, f"{os.path.relpath(exe, REPO)} failed: {e}"
     m = re.search(r"Normsq result: (\S+)\s+diff: (\S+)\s+(\w+)\s+time: (\S+)\s+GF/s: (\S+)", out)
     if not m:
-        return None
+        return None, f"{os.path.relpath(exe, REPO)} printed no result line"
     return {
         "value": float(m.group(5)),
         "unit": "GFLOP/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "reference",
-        "sample": (f"reference serial KBA sweep, {ncx}x{ncy}x{ncz} cells, ne=1 group (of the "
-                   f"workload's energy groups), na={na}, nm={nm}, nu=4, 1 sweep, "
+        "sample": (f"reference {build} sweep ({os.path.relpath(exe, REPO)}, built from the "
+                   f"reference's sources by oracle/Makefile), {ncx}x{ncy}x{ncz} cells, ne={ne} "
+                   f"group(s) of the workload's, na={na}, nm={nm}, nu=4, 1 sweep, "
                    f"{float(m.group(4)):.1f} s, {m.group(3)}"),
-    }
+    }, None
 
 
 def pct(vals, p):
@@ -192,11 +209,24 @@ def main():
               "only the one-GPU share is built)", file=sys.stderr)
         sys.exit(2)
 
-    # the reference's CPU run goes first, before this process touches the GPU
-    ref_cpu = None
+    # the reference's CPU runs go first, before this process touches the GPU:
+    # its OpenMP-threads build on the host's cores (the reported baseline) and
+    # its serial build on one core
+    ref_cpu = ref_serial = None
+    cpu_threads = max(1, min(args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "16")),
+                             16))
+    missing = []
     if not args.no_cpu_baseline:
-        ref_cpu = cpu_baseline_reference(args.ncell, args.ncell, args.ncell, args.nm, args.na,
-                                         args.nu)
+        ref_cpu, why = cpu_baseline_reference(args.ncell, args.ncell, args.ncell, args.nm,
+                                              args.na, args.nu, cpu_threads)
+        if why:
+            missing.append(why)
+        ref_serial, why = cpu_baseline_reference(args.ncell, args.ncell, args.ncell, args.nm,
+                                                 args.na, args.nu, 1)
+        if why:
+            missing.append(why)
+        for w in missing:
+            print(f"bench.py: cpu_baseline: {w}", file=sys.stderr)
 
     import torch
     import minisweep_amd as msw
@@ -245,9 +275,10 @@ def main():
     # reference runner's loop, src/4_driver/runner.c:148-155) the first NIT-1
     # run in mixed FP32 and the last in FP64.  Timed once on the manufactured
     # input (exactness checked).  The error is taken over NIT_ERR = 2
-    # iterations of the perturbed input (mixed then FP64, against FP64 twice
-    # and against mixed twice): each sweep multiplies the perturbation by
-    # about 5e5, so by the fourth iteration the values leave FP32's range.
+    # iterations of the perturbed input (mixed then FP64, against FP64
+    # twice): at 64^3 one sweep takes the perturbation from 2^-8 to values
+    # near 1e29 and a second to 6e56, past FP32's range, so two mixed
+    # iterations of it are not a measurement.
     final = None
     if len(forms) > 1 and "fp32" in forms:
         nit = 4
@@ -279,8 +310,7 @@ def main():
         n64, _ = msw.state_norms(d, x0, r64)
         r64_max = float(r64.abs().max())
         errs_final = {}
-        for name, plist in (("fp64_last", [plans["fp32"]] * (nit_err - 1) + [plans["fp64"]]),
-                            ("all_mixed", [plans["fp32"]] * nit_err)):
+        for name, plist in (("fp64_last", [plans["fp32"]] * (nit_err - 1) + [plans["fp64"]]),):
             rm, _ = chain(plist, x0, vo, ref)
             nm_, _ = msw.state_norms(d, x0, rm)
             errs_final[name] = {"normsq_rel": abs(nm_ - n64) / n64,
@@ -296,8 +326,6 @@ def main():
             "error_iterations": nit_err,
             "normsq_rel_err_vs_fp64_perturbed": errs_final["fp64_last"]["normsq_rel"],
             "vo_normwise_err_vs_fp64_perturbed": errs_final["fp64_last"]["vo_normwise"],
-            "all_mixed_normsq_rel_err_vs_fp64_perturbed": errs_final["all_mixed"]["normsq_rel"],
-            "all_mixed_vo_normwise_err_vs_fp64_perturbed": errs_final["all_mixed"]["vo_normwise"],
         }
         del x0, out64, r64
 
@@ -342,12 +370,16 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "16"))
-        threads = max(1, min(threads, 16))
-        cpu = cpu_baseline(n, n, n, args.nm, args.na, args.nu, threads)
+        port = cpu_baseline(n, n, n, args.nm, args.na, args.nu, cpu_threads)
         if ref_cpu is not None:
-            ref_cpu["port"] = cpu
             cpu = ref_cpu
+            cpu["port"] = port
+            cpu["port_over_reference"] = round(port["value"] / ref_cpu["value"], 3)
+        else:
+            cpu = port
+            cpu["reference_missing"] = "; ".join(missing)
+        if ref_serial is not None:
+            cpu["reference_serial"] = ref_serial
 
     out = {
         "metric": METRIC,

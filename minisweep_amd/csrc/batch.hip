@@ -215,16 +215,22 @@ void choose_tick_epw(std::vector<TickWork>& work, int precision) {
 }
 
 // Host-side checks of one problem's share of a tick before it goes into a
-// MultiLaunch: the launch exists in its schedule, the stream length is
-// positive, and its workgroups fit the kernel's int indexing.  Returns the
-// workgroup count, or -1 (and sets the error text).
-long long tick_workgroups(const TickWork& w) {
+// MultiLaunch: the launch exists in its schedule, the stream length fits its
+// bits, and its workgroups fit the kernel's int indexing.  Returns the
+// (item, group chunk) count, or -1 (and sets the error text); its workgroups
+// are xcd_round_ids(count, nu).
+long long tick_groups(const TickWork& w) {
   if (!w.g || !w.s || w.launch < 0 || w.launch >= w.s->nlaunch() || w.epw < 1 ||
-      w.s->cnt[w.launch] < 1 || w.s->off[w.launch] < 0) {
+      w.epw > kEpwMask || w.s->cnt[w.launch] < 1 || w.s->off[w.launch] < 0) {
     set_error(MSW_ERR_INVALID, "internal: tick work outside its schedule");
     return -1;
   }
-  return (long long)w.s->cnt[w.launch] * ((w.g->d.ne + w.epw - 1) / w.epw) * w.g->d.nu;
+  const long long ng = (long long)w.s->cnt[w.launch] * ((w.g->d.ne + w.epw - 1) / w.epw);
+  if (ng >= (1ll << (31 - kEpwBits))) {
+    set_error(MSW_ERR_UNSUPPORTED, "internal: tick segment of too many workgroups");
+    return -1;
+  }
+  return ng;
 }
 
 // Enqueue one combined tick: per kernel variant, k_sweep_mfma_multi launches
@@ -245,8 +251,9 @@ int enqueue_tick(int precision, std::vector<TickWork>& work, const SweepParams* 
       long long grid = 0;
       for (size_t k = b; k < ws.size() && k < b + kMaxSegments; ++k) {
         const TickWork& w = *ws[k];
-        const long long nwg = tick_workgroups(w);
-        if (nwg < 0) return -MSW_ERR_INVALID;
+        const long long ng = tick_groups(w);
+        if (ng < 0) return -MSW_ERR_INVALID;
+        const long long nwg = xcd_round_ids(ng, w.g->d.nu);
         if (ml.nseg >= kMaxSegments || w.slot < 0) {
           set_error(MSW_ERR_INVALID, "internal: segment table overflow");
           return -MSW_ERR_INVALID;
@@ -254,7 +261,7 @@ int enqueue_tick(int precision, std::vector<TickWork>& work, const SweepParams* 
         ml.start[ml.nseg] = (int)grid;
         ml.prob[ml.nseg] = w.slot;
         ml.item_base[ml.nseg] = w.s->off[w.launch];
-        ml.epw[ml.nseg] = w.epw;
+        ml.epw_ng[ml.nseg] = w.epw | (int)ng << kEpwBits;
         ml.nseg++;
         grid += nwg;
       }

@@ -102,14 +102,25 @@ struct SweepParams {
 // A launch that runs segments of several sweeps: segment k covers
 // workgroups [start[k], start[k+1]) (the last one up to the grid size) and is
 // launch-local work of problem prob[k], whose items start at item_base[k].
-// Passed by value as a kernel argument.
+// epw_ng[k] packs the energy groups streamed through one workgroup (low
+// kEpwBits bits) and the segment's (item, group chunk) count above them; a
+// segment's workgroup count is that count padded to whole rounds of the XCDs
+// (xcd_round_ids).  Passed by value as a kernel argument.
 constexpr int kMaxSegments = 128;
+constexpr int kEpwBits = 6;
+constexpr int kEpwMask = (1 << kEpwBits) - 1;
 struct MultiLaunch {
   int nseg;
   int start[kMaxSegments];
   int prob[kMaxSegments];
   int item_base[kMaxSegments];
-  int epw[kMaxSegments];  // energy groups streamed through one workgroup
+  int epw_ng[kMaxSegments];
 };
+
+// Workgroups of a launch of ngroups (item, group chunk) pairs of nu unknowns
+// each, padded so that all unknowns of a pair share one of the 8 XCDs.
+__host__ __device__ __forceinline__ long long xcd_round_ids(long long ngroups, int nu) {
+  return (ngroups + 7) / 8 * 8 * nu;
+}
 
 }  // namespace msw

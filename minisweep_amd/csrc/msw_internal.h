@@ -22,6 +22,7 @@ struct LaunchSpec {
   int epw;           // energy groups streamed through one workgroup (MFMA)
   int ny;            // row groups of 4 rows per tile (MFMA)
   size_t lds_bytes;  // dynamic LDS per workgroup
+  int ngroups;       // (item, group chunk) pairs (MFMA: grid = xcd_round_ids(ngroups, nu))
 };
 
 // Sweep with m2a/a2m on MFMA (sweep_mfma.hip), in the plan's precision

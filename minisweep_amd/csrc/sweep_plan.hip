@@ -491,6 +491,8 @@ static int launch_all(MswPlan* p, const SweepParams& sp, hipStream_t st, size_t 
       ls.nm = p->d.nm;
       ls.lds_bytes = p->lds_bytes;
       ls.epw = epw;
+      ls.ngroups = (int)p->launch_cnt[l] * nec;
+      ls.grid = (unsigned)xcd_round_ids(ls.ngroups, p->d.nu);
       launch_mfma(p->precision, ls, sp, st);
     } else {
       hipLaunchKernelGGL((k_sweep_fp64<NM, NAC>), grid, block, p->lds_bytes, st, sp,

@@ -888,12 +888,29 @@ __device__ __forceinline__ void sweep_block(const SweepParams& p, int item_base,
                                        iu);
 }
 
+// Workgroup ids and the 8 XCDs.  The dispatcher deals workgroups to the XCDs
+// round-robin by id, and each XCD has its own L2.  The nu unknowns of one
+// (item, group chunk) G read and write the same cache lines (a cell's
+// unknowns are adjacent in the state and in the mixed forms' FP32 and BF16
+// copies), so they get ids on one XCD: id = ((G / 8) nu + iu) 8 + G % 8.  A
+// launch of ng groups is padded to xcd_round_ids(ng, nu) ids; the ids past the
+// last group exit at once.  Returns the (G nu + iu) index sweep_block takes,
+// or -1.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_unmap(int id, int nu, int ngroups) {
+  const int k = id / kXcds;
+  const int g = (k / nu) * kXcds + id % kXcds;
+  return g < ngroups ? g * nu + k % nu : -1;
+}
+
 // Single sweep (ZB: one z block of it).
 template <class PR, int NM, int NAC, int NWA, int NY, bool ZB>
 __global__ __launch_bounds__(64 * NWA * NY) __attribute__((amdgpu_waves_per_eu(
     Occupancy<PR, NM, NAC, NWA, NY>::value))) void k_sweep_mfma(SweepParams p, int item_base,
-                                                                int epw) {
-  sweep_block<PR, NM, NAC, NWA, NY, ZB>(p, item_base, (int)blockIdx.x, epw);
+                                                                int epw, int ngroups) {
+  const int w = xcd_unmap((int)blockIdx.x, p.nu, ngroups);
+  if (w < 0) return;
+  sweep_block<PR, NM, NAC, NWA, NY, ZB>(p, item_base, w, epw);
 }
 
 // Many sweeps in one launch: segment k of the launch is one launch of the
@@ -908,7 +925,10 @@ __global__ __launch_bounds__(64 * NWA * NY) __attribute__((amdgpu_waves_per_eu(
   for (int step = kMaxSegments / 2; step > 0; step >>= 1)
     if (k + step < ml.nseg && ml.start[k + step] <= blk) k += step;
   const SweepParams p = probs[ml.prob[k]];
-  sweep_block<PR, NM, NAC, NWA, NY>(p, ml.item_base[k], blk - ml.start[k], ml.epw[k]);
+  const int epw = ml.epw_ng[k] & kEpwMask;
+  const int w = xcd_unmap(blk - ml.start[k], p.nu, ml.epw_ng[k] >> kEpwBits);
+  if (w < 0) return;
+  sweep_block<PR, NM, NAC, NWA, NY>(p, ml.item_base[k], w, epw);
 }
 
 // Workgroup shapes built: (waves of angle chunks, row groups of 4 rows), at
@@ -923,12 +943,12 @@ static void launch_nac(const LaunchSpec& ls, const SweepParams& sp, hipStream_t 
     if constexpr (std::is_same<PR, PrecF64>::value) {                                     \
       if (sp.fz) {                                                                         \
         hipLaunchKernelGGL((k_sweep_mfma<PR, NM, NAC, W, Y, true>), dim3(ls.grid), block,   \
-                           ls.lds_bytes, st, sp, ls.item_base, ls.epw);                   \
+                           ls.lds_bytes, st, sp, ls.item_base, ls.epw, ls.ngroups);       \
         return;                                                                            \
       }                                                                                    \
     }                                                                                      \
     hipLaunchKernelGGL((k_sweep_mfma<PR, NM, NAC, W, Y, false>), dim3(ls.grid), block,      \
-                       ls.lds_bytes, st, sp, ls.item_base, ls.epw);                       \
+                       ls.lds_bytes, st, sp, ls.item_base, ls.epw, ls.ngroups);           \
     return;                                                                                \
   }
   MSW_SHAPES(MSW_CASE)

@@ -156,9 +156,18 @@ void launch_mfma_multi(int precision, int nm, int nac, int nwa, int ny, unsigned
       if (m.start[0] != 0) DIE("segment 0 starts at %d", m.start[0]);
       check_range(probs + m.prob[k], sizeof(SweepParams), "probs[prob[k]]");
       const SweepParams p = probs[m.prob[k]];
-      int bid = (int)blk - m.start[k];
-      const int epw = m.epw[k];
+      const int epw = m.epw_ng[k] & kEpwMask;
+      const int ng = m.epw_ng[k] >> kEpwBits;
       if (epw < 1 || epw > kMaxStreamGroups) DIE("epw %d", epw);
+      const long long seg_end = k + 1 < m.nseg ? m.start[k + 1] : (long long)grid;
+      if (seg_end - m.start[k] != xcd_round_ids(ng, p.nu)) DIE("segment %d: %lld workgroups for %d groups", k, seg_end - m.start[k], ng);
+      if (m.start[k] % 8) DIE("segment %d starts off an XCD round", k);
+      int bid = (int)blk - m.start[k];
+      {  // the kernel's XCD mapping: id -> (group, unknown), ids past the groups exit
+        const int kk = bid / 8, g = (kk / p.nu) * 8 + bid % 8;
+        if (g >= ng) { ++g_checked_wg; continue; }
+        bid = g * p.nu + kk % p.nu;
+      }
       const int iu = bid % p.nu; bid /= p.nu;
       const int nec = (p.ne + epw - 1) / epw;
       const int ec = bid % nec; bid /= nec;
