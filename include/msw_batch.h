@@ -44,6 +44,10 @@ typedef struct {
                         reference's octant order */
   int stagger;       /* 1: start problems at offsets that even out the work of
                         each combined launch; 0: all start together */
+  int row_groups;    /* rows of every tile in groups of 4: 1, 2 or 4 (0: the
+                        plan default of the precision) */
+  int stream_target; /* z-stream steps per workgroup a tick aims for: energy
+                        groups streamed = target / ncell_z (0: 128) */
 } MswBatchOptions;
 
 void msw_batch_default_options(MswBatchOptions* opt);

@@ -45,7 +45,8 @@ class MswPlanOptions(ctypes.Structure):
 
 class MswBatchOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("schedule_kind", ctypes.c_int), ("stagger", ctypes.c_int)]
+                ("schedule_kind", ctypes.c_int), ("stagger", ctypes.c_int),
+                ("row_groups", ctypes.c_int), ("stream_target", ctypes.c_int)]
 
 
 class MswServerOptions(ctypes.Structure):
@@ -277,11 +278,13 @@ class Batch:
     per run, kernel variants chosen per (nm, na) class, optionally autotuned."""
 
     def __init__(self, dims_list, precision: int = PREC_FP64, use_graph: bool = True,
-                 schedule_kind: int = 0, stagger: bool = True):
+                 schedule_kind: int = 0, stagger: bool = True, row_groups: int = 0,
+                 stream_target: int = 0):
         o = MswBatchOptions()
         lib().msw_batch_default_options(ctypes.byref(o))
         o.precision, o.use_graph = precision, int(use_graph)
         o.schedule_kind, o.stagger = schedule_kind, int(stagger)
+        o.row_groups, o.stream_target = row_groups, stream_target
         arr = (MswDims * len(dims_list))(*dims_list)
         h = ctypes.c_void_p()
         _check(lib().msw_batch_create(len(dims_list), arr, ctypes.byref(o), ctypes.byref(h)),

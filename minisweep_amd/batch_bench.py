@@ -3,10 +3,14 @@ sequential loop of plain FP64 sweeps of the same problems.
 
 Problems: 256 drawn from seed 0 (minisweep_amd.workloads.uneven_problems),
 manufactured input state (Quantities_init_state), one sweep each.  The batch
-runs the FP32-MFMA mixed form by default, as one hipGraph replay, in its
-default configuration (--autotune, with MSW_BATCH_AUTOTUNE=1 in the
-environment, lets the autotuner pick angles per wave per (nm, na) class and
-the schedule heuristic first); the FP64 batch is timed too, to separate
+runs the FP32-MFMA mixed form by default, as one hipGraph replay, in the
+configuration given by --schedule-kind, --stagger, --row-groups and
+--stream-target.  Their defaults are the configuration the autotuner settled
+on for this workload in its last run (the reference's octant order, no
+stagger, one row group of 4 rows per tile, 128-step streams, the default
+angles per wave), set by hand: the tuner itself is off (msw_batch.h).
+--autotune, with MSW_BATCH_AUTOTUNE=1 in the environment, runs it first.  The
+FP64 batch is timed too, in the library's default configuration, to separate
 batching from precision.
 Every result is checked: normsq equals the closed form and normsqdiff is 0.
 
@@ -29,6 +33,10 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--precision", choices=["fp32", "bf16", "fp64"], default="fp32")
+    ap.add_argument("--schedule-kind", type=int, default=1)
+    ap.add_argument("--stagger", type=int, default=0)
+    ap.add_argument("--row-groups", type=int, default=1)
+    ap.add_argument("--stream-target", type=int, default=128)
     ap.add_argument("--autotune", action="store_true",
                     help="autotune the batch first (needs MSW_BATCH_AUTOTUNE=1)")
     args = ap.parse_args()
@@ -85,8 +93,10 @@ def main():
     for p in plans:
         p.close()
 
-    def batch_of(precision, tune):
-        b = msw.Batch(dims, precision=precision)
+    def batch_of(precision, tune, configured=False):
+        kw = dict(schedule_kind=args.schedule_kind, stagger=bool(args.stagger),
+                  row_groups=args.row_groups, stream_target=args.stream_target) if configured else {}
+        b = msw.Batch(dims, precision=precision, **kw)
         for i, d in enumerate(dims):
             a, m = mats[(d.nm, d.na)]
             b.set_problem(i, vis[i], vos[i], a, m)
@@ -102,7 +112,7 @@ def main():
     bad_b64 = check()
     b64.close()
 
-    bm, t_tune = batch_of(prec, args.autotune)
+    bm, t_tune = batch_of(prec, args.autotune, configured=True)
     t_bm = time_runs(bm.run, args.reps)
     bad_bm = check()
     desc = bm.describe()

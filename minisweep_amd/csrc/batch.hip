@@ -592,6 +592,8 @@ void msw_batch_default_options(MswBatchOptions* o) {
   o->use_graph = 1;
   o->schedule_kind = 0;
   o->stagger = 1;
+  o->row_groups = 0;
+  o->stream_target = 0;
 }
 
 int msw_batch_create(int nprob, const MswDims* dims, const MswBatchOptions* opt, MswBatch** out) {
@@ -604,6 +606,10 @@ int msw_batch_create(int nprob, const MswDims* dims, const MswBatchOptions* opt,
     return set_error(MSW_ERR_INVALID, "unknown precision");
   if (o.schedule_kind != 0 && o.schedule_kind != 1)
     return set_error(MSW_ERR_INVALID, "schedule_kind must be 0 or 1");
+  if (o.row_groups != 0 && o.row_groups != 1 && o.row_groups != 2 && o.row_groups != 4)
+    return set_error(MSW_ERR_INVALID, "row_groups must be 0, 1, 2 or 4");
+  if (o.stream_target < 0 || o.stream_target > 4096)
+    return set_error(MSW_ERR_INVALID, "stream_target must be 0..4096");
   for (int i = 0; i < nprob; ++i) {
     Geom g;
     const int rc = make_geom(dims[i], default_mfma_nac(o.precision, dims[i].na), &g);
@@ -611,6 +617,8 @@ int msw_batch_create(int nprob, const MswDims* dims, const MswBatchOptions* opt,
   }
   MswBatch* b = new MswBatch();
   b->opt = o;
+  b->row_groups = o.row_groups;
+  b->stream_target = o.stream_target ? o.stream_target : 128;
   b->dims.assign(dims, dims + nprob);
   b->probs.resize(nprob);
   hipError_t e = hipEventCreateWithFlags(&b->done_ev, hipEventDisableTiming);

@@ -608,11 +608,11 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
 #pragma unroll
   for (int b = 0; b < kTY; ++b) exs[b] = scale_exp(b, zs[b], es[b]);
   scale_vi(PR::kPingPong ? vbA : vbB, exs);
-  // BF16: m2a of angle block 0 for the coming step is issued at the end of
-  // the step before, so the matrix cores run it during that step's vo
-  // reduction (measured at 64^3: FP64 runs slower that way, 120 vs 90 ms,
-  // FP32 the same)
-  constexpr bool kEarlyM2a = PR::kPacked;
+  // Mixed forms: m2a of angle block 0 for the coming step is issued at the
+  // end of the step before, so the matrix cores run it during that step's vo
+  // reduction (measured at 64^3: FP32 kernel 48.1 -> 46.4 ms, BF16 41.6 ->
+  // 40.0 ms when it was introduced; FP64 runs slower that way, 120 vs 90 ms)
+  constexpr bool kEarlyM2a = !kF64;
   acc4 vs0[kTY];
   if constexpr (kEarlyM2a) m2a(0, PR::kPingPong ? vbA : vbB, vs0);
 

@@ -13,11 +13,14 @@ Latency = completion - submission on the server clock; throughput = requests
 submitted at once) is measured for both.  Every result is checked against the
 closed-form normsq.
 
-By default the server runs the plan defaults.  With --tune (and
-MSW_BATCH_AUTOTUNE=1 in the environment) the served form is tuned once first:
-a batch of --tune-problems problems from the same generator (seed 0, the
-batched workload) is autotuned and the server takes its kernel configuration
-(msw_server_tune_from_batch).
+The served form takes its kernel configuration from a batch of the batched
+workload's shapes (msw_server_tune_from_batch): by default a batch configured
+by hand with --schedule-kind, --stagger, --row-groups and --stream-target
+(their defaults are what the autotuner settled on for the batched workload in
+its last run; the tuner itself is off, msw_batch.h).  With --tune (and
+MSW_BATCH_AUTOTUNE=1 in the environment) that batch of --tune-problems
+problems (seed 0) is autotuned first.  --plan-defaults serves with the plan
+defaults instead.
 
     python -m minisweep_amd.serve_bench [--rates 50,200,800] [--requests 10000]
 
@@ -127,6 +130,11 @@ def main():
     ap.add_argument("--max-active", type=int, default=256)
     ap.add_argument("--precision", choices=["fp32", "bf16", "fp64"], default="fp32")
     ap.add_argument("--skip-baseline", action="store_true")
+    ap.add_argument("--schedule-kind", type=int, default=1)
+    ap.add_argument("--stagger", type=int, default=0)
+    ap.add_argument("--row-groups", type=int, default=1)
+    ap.add_argument("--stream-target", type=int, default=128)
+    ap.add_argument("--plan-defaults", action="store_true")
     ap.add_argument("--tune", action="store_true",
                     help="autotune on the batched workload first (needs MSW_BATCH_AUTOTUNE=1)")
     ap.add_argument("--tune-problems", type=int, default=256)
@@ -154,6 +162,12 @@ def main():
     if args.tune:
         progress(f"autotuning the served form on {args.tune_problems} problems")
         tune, bufs = tuned_batch(msw, prec, args.tune_problems)
+    elif not args.plan_defaults:
+        tune = msw.Batch([msw.dims(*s) for s in uneven_problems(args.tune_problems, 0)],
+                         precision=prec, schedule_kind=args.schedule_kind,
+                         stagger=bool(args.stagger), row_groups=args.row_groups,
+                         stream_target=args.stream_target)
+    if tune is not None:
         tuned_desc = tune.describe()
         tuned_desc.pop("tuning", None)
 

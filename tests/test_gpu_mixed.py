@@ -117,3 +117,81 @@ def test_mixed_deterministic_and_chunking(prec, cuda):
     np.testing.assert_array_equal(r1, r2)
     r3, _, _ = _sweep(d, vi0, prec, torch, cuda, nac=64)
     np.testing.assert_array_equal(r3, vi0)  # exact either way on this input
+
+
+MODEL_SHAPES = [(6, 5, 7, 2, 16, 64, 2), (9, 7, 3, 2, 16, 64, 4), (7, 3, 4, 1, 8, 40, 3)]
+# GPU vs its own rounding model (oracle/mixed_model.py): the model's two
+# accumulation variants (one rounding per block sum, or the products aligned
+# and truncated as a matrix core may) agree to ~1e-8 normwise in BF16, so
+# the bars are loose against that and tight against the O(1) a wrong index
+# gives; the BF16 form itself is O(1) from FP64 on this input.
+MODEL_VO_TOL = {msw.PREC_MIXED_FP32: 1e-4, msw.PREC_MIXED_BF16: 1e-3}
+MODEL_NORMSQ_TOL = {msw.PREC_MIXED_FP32: 1e-5, msw.PREC_MIXED_BF16: 1e-4}
+
+
+@pytest.mark.parametrize("prec", PRECS, ids=PREC_IDS)
+@pytest.mark.parametrize("shape", MODEL_SHAPES, ids=[str(s) for s in MODEL_SHAPES])
+def test_mixed_matches_rounding_model(shape, prec, cuda):
+    """Each mixed form against a CPU model of its own roundings (BF16 or FP32
+    operands, FP32 solve and sums), perturbed input: pins the BF16 form, whose
+    error against FP64 is O(1), to the arithmetic it states."""
+    import torch
+    from oracle import mixed_model
+    print(f"case model {shape} prec {prec}", flush=True)
+    d = msw.dims(*shape)
+    od = oracle.dims(*shape)
+    rng = np.random.default_rng(1234)
+    base = oracle.init_state(od)
+    vi0 = base * (1.0 + rng.uniform(-2.0 ** -8, 2.0 ** -8, base.size))
+    vo, n, nd = _sweep(d, vi0, prec, torch, cuda)
+    a, m = oracle.init_matrices(od)
+    form = "bf16" if prec == msw.PREC_MIXED_BF16 else "fp32"
+    ref = mixed_model.sweep(shape, a, m, vi0, form)
+    err = _errs(vo, ref)
+    n_ref = float(np.sum(ref * ref))
+    rel_n = abs(float(np.sum(vo * vo)) - n_ref) / n_ref
+    print(f"  vo normwise err vs model {err:.3e}  normsq rel err {rel_n:.3e}", flush=True)
+    assert err <= MODEL_VO_TOL[prec]
+    assert rel_n <= MODEL_NORMSQ_TOL[prec]
+
+
+def test_mixed_then_fp64_iterations(cuda):
+    """The mixed form with a final FP64 sweep: of the runner's ping-pong
+    iterations the first ones in FP32, the last in FP64.  Manufactured input:
+    four iterations exact.  Perturbed input: two iterations (FP32, FP64)
+    against the oracle's two FP64 iterations within the FP32 form's 1e-5
+    normsq bar (a second FP32 iteration of this input leaves FP32's range on
+    large grids: each sweep multiplies the perturbation by orders of
+    magnitude)."""
+    import torch
+    shape = (9, 7, 5, 2, 16, 64, 4)
+    print(f"case mixed then fp64 {shape}", flush=True)
+    d = msw.dims(*shape)
+    od = oracle.dims(*shape)
+    a, m = msw.init_matrices(d.nm, d.na)
+    ta, tm = torch.from_numpy(a).to(cuda), torch.from_numpy(m).to(cuda)
+    p32 = msw.Plan(d, msw.PREC_MIXED_FP32, engine=msw.ENGINE_MFMA)
+    p64 = msw.Plan(d, msw.PREC_FP64, engine=msw.ENGINE_MFMA)
+
+    def run(plans, x0):
+        u = torch.from_numpy(np.ascontiguousarray(x0)).to(cuda)
+        w = torch.full_like(u, float("nan"))
+        for pl in plans:
+            pl.sweep(u, w, ta, tm)
+            u, w = w, u
+        torch.cuda.synchronize()
+        return u.cpu().numpy()
+
+    base = oracle.init_state(od)
+    np.testing.assert_array_equal(run([p32, p32, p32, p64], base), base)
+    rng = np.random.default_rng(7)
+    x0 = base * (1.0 + rng.uniform(-2.0 ** -8, 2.0 ** -8, base.size))
+    got = run([p32, p64], x0)
+    oa, om = oracle.init_matrices(od)
+    ref = oracle.sweep(od, oa, om, oracle.sweep(od, oa, om, x0))
+    n_ref = float(np.sum(ref * ref))
+    rel_n = abs(float(np.sum(got * got)) - n_ref) / n_ref
+    print(f"  two iterations: normsq rel err {rel_n:.3e}", flush=True)
+    assert rel_n <= 1e-5
+    p32.close()
+    p64.close()

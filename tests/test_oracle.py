@@ -218,3 +218,60 @@ def test_fp64_error_on_ill_conditioned_input():
     from parity import elem_err, normwise_err
     assert elem_err(f64, ld) > 1e-12
     assert normwise_err(f64, ld) < 2e-13
+
+
+# ---- the mixed forms' rounding model (oracle/mixed_model.py) ---------------
+
+MODEL_SHAPES = [(5, 4, 6, 2, 16, 64, 2), (7, 3, 4, 1, 8, 40, 3), (3, 6, 5, 2, 4, 13, 1)]
+
+
+@pytest.mark.parametrize("shape", MODEL_SHAPES, ids=[str(s) for s in MODEL_SHAPES])
+def test_mixed_model_without_rounding_is_the_oracle(shape):
+    """With every rounding off the model is the reference algorithm, pinned to
+    the oracle on perturbed input: 1e-10 per element (with the 1e-2 max floor)
+    and 1e-13 normwise.  The model sums in the carried form and by 16-angle
+    blocks, the oracle in the reference's order; on these inputs the two
+    orders differ by up to ~3e-12 per element.  A wrong face weight, octant
+    direction or matrix index gives O(1) errors."""
+    from oracle import mixed_model
+    from parity import elem_err, normwise_err
+    d = oracle.dims(*shape)
+    a, m = oracle.init_matrices(d)
+    base = oracle.init_state(d)
+    rng = np.random.default_rng(5)
+    vi = base * (1.0 + rng.uniform(-2.0 ** -8, 2.0 ** -8, base.size))
+    got = mixed_model.sweep(shape, a, m, vi, "fp64")
+    ref = oracle.sweep(d, a, m, vi)
+    assert elem_err(got, ref) <= 1e-10
+    assert normwise_err(got, ref) <= 1e-13
+
+
+@pytest.mark.parametrize("form", ["fp32", "bf16"])
+def test_mixed_model_manufactured_exact(form):
+    """Every operand of the manufactured problem is a small integer times a
+    power of two: the rounded forms reproduce vo == vi exactly."""
+    from oracle import mixed_model
+    shape = (5, 4, 6, 2, 16, 64, 2)
+    d = oracle.dims(*shape)
+    a, m = oracle.init_matrices(d)
+    vi = oracle.init_state(d)
+    np.testing.assert_array_equal(mixed_model.sweep(shape, a, m, vi, form), vi)
+
+
+def test_mixed_model_error_levels():
+    """The model's errors against FP64 on perturbed input are the levels the
+    GPU forms are held to: FP32 within 1e-5 in normsq; BF16 O(1) (its
+    operands are rounded to 2^-9 before the matrices' cancellation)."""
+    from oracle import mixed_model
+    shape = (6, 5, 7, 2, 16, 64, 2)
+    d = oracle.dims(*shape)
+    a, m = oracle.init_matrices(d)
+    base = oracle.init_state(d)
+    rng = np.random.default_rng(1234)
+    vi = base * (1.0 + rng.uniform(-2.0 ** -8, 2.0 ** -8, base.size))
+    ref = oracle.sweep(d, a, m, vi)
+    n_ref = float(np.sum(ref * ref))
+    v32 = mixed_model.sweep(shape, a, m, vi, "fp32")
+    assert abs(float(np.sum(v32 * v32)) - n_ref) / n_ref <= 1e-5
+    v16 = mixed_model.sweep(shape, a, m, vi, "bf16")
+    assert np.max(np.abs(v16 - ref)) / np.max(np.abs(ref)) > 0.1
