@@ -48,6 +48,9 @@ typedef struct {
                         plan default of the precision) */
   int stream_target; /* z-stream steps per workgroup a tick aims for: energy
                         groups streamed = target / ncell_z (0: 128) */
+  int xcd_map;       /* 1 (default): the unknowns of a tile get workgroup ids on
+                        one XCD, as in a single sweep (each problem's launch
+                        padded to whole rounds of the 8 XCDs); 0: ids in order */
 } MswBatchOptions;
 
 void msw_batch_default_options(MswBatchOptions* opt);
@@ -99,6 +102,7 @@ typedef struct {
   int schedule_kind;        /* tile schedule heuristic, as MswPlanOptions (0 most
                                remaining octant first, 1 the reference's octant
                                order); msw_server_tune_from_batch replaces it */
+  int xcd_map;              /* as MswBatchOptions.xcd_map (default 1) */
 } MswServerOptions;
 
 typedef struct {

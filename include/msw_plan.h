@@ -109,6 +109,22 @@ int msw_plan_sweep(MswPlan* plan, const double* vi, double* vo,
                    const double* a_from_m, const double* m_from_a,
                    void* stream);
 
+/* msw_plan_sweep that also writes each octant's outflow faces at the
+ * domain's downstream boundary, in the reference's Faces layouts with all
+ * eight octants (src/3_sweeper/array_accessors_kernels.h:481-486, 545-550,
+ * 609-614; the reference's Faces of faces_kba.h:29-142 with
+ * noctant_per_block 8), FP64 device arrays:
+ *   facexy[8][ne][ny][nx][nu][na]  the z faces: each (x, y) column's last cell
+ *   facexz[8][ne][nz][nx][nu][na]  the y faces: each (x, z) row's last cell
+ *   faceyz[8][ne][nz][ny][nu][na]  the x faces: each (y, z) row's last cell
+ * "last" in the octant's direction; each value is the solved angular value
+ * times (1+octant), as the reference stores its faces
+ * (src/3_sweeper/quantities_testing_kernels.h:437-443).  MFMA engine, plans
+ * without z blocks (else MSW_ERR_UNSUPPORTED). */
+int msw_plan_sweep_faces(MswPlan* plan, const double* vi, double* vo,
+                         const double* a_from_m, const double* m_from_a, double* facexy,
+                         double* facexz, double* faceyz, void* stream);
+
 /* Out-of-core sweep (this library; the reference's z-block streaming,
  * src/3_sweeper/sweeper_kba_c.h:719-781): vi_host and vo_host are HOST arrays
  * in the state layout, pinned (hipHostMalloc) for the copies to overlap the

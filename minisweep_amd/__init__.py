@@ -46,13 +46,14 @@ class MswPlanOptions(ctypes.Structure):
 class MswBatchOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("schedule_kind", ctypes.c_int), ("stagger", ctypes.c_int),
-                ("row_groups", ctypes.c_int), ("stream_target", ctypes.c_int)]
+                ("row_groups", ctypes.c_int), ("stream_target", ctypes.c_int),
+                ("xcd_map", ctypes.c_int)]
 
 
 class MswServerOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("hbm_budget_bytes", ctypes.c_size_t),
                 ("max_active", ctypes.c_int), ("keep_outputs", ctypes.c_int),
-                ("schedule_kind", ctypes.c_int)]
+                ("schedule_kind", ctypes.c_int), ("xcd_map", ctypes.c_int)]
 
 
 class MswResult(ctypes.Structure):
@@ -93,6 +94,7 @@ def lib():
         L.msw_plan_num_launches.argtypes = [vp]
         L.msw_plan_sweep.argtypes = [vp, vp, vp, vp, vp, vp]
         L.msw_plan_sweep_host.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.msw_plan_sweep_faces.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.msw_plan_destroy.argtypes = [vp]
         L.msw_plan_set_timing.argtypes = [vp, ctypes.c_int]
         L.msw_plan_last_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_float),
@@ -213,6 +215,14 @@ class Plan:
                                     _dptr(m_from_a), _stream_ptr(stream)),
                "msw_plan_sweep")
 
+    def sweep_faces(self, vi, vo, a_from_m, m_from_a, facexy, facexz, faceyz, stream=None):
+        """sweep() that also writes the outflow faces at the downstream
+        boundary in the reference's Faces layouts (msw_plan_sweep_faces)."""
+        _check(lib().msw_plan_sweep_faces(self._h, _dptr(vi), _dptr(vo), _dptr(a_from_m),
+                                          _dptr(m_from_a), _dptr(facexy), _dptr(facexz),
+                                          _dptr(faceyz), _stream_ptr(stream)),
+               "msw_plan_sweep_faces")
+
     def sweep_host(self, vi_host, vo_host, a_from_m, m_from_a, stream=None):
         """vo_host <- S(vi_host) with the state in host memory (CPU tensors,
         pinned for overlap), streamed through HBM by z block; the matrices are
@@ -279,12 +289,13 @@ class Batch:
 
     def __init__(self, dims_list, precision: int = PREC_FP64, use_graph: bool = True,
                  schedule_kind: int = 0, stagger: bool = True, row_groups: int = 0,
-                 stream_target: int = 0):
+                 stream_target: int = 0, xcd_map: bool = True):
         o = MswBatchOptions()
         lib().msw_batch_default_options(ctypes.byref(o))
         o.precision, o.use_graph = precision, int(use_graph)
         o.schedule_kind, o.stagger = schedule_kind, int(stagger)
         o.row_groups, o.stream_target = row_groups, stream_target
+        o.xcd_map = int(xcd_map)
         arr = (MswDims * len(dims_list))(*dims_list)
         h = ctypes.c_void_p()
         _check(lib().msw_batch_create(len(dims_list), arr, ctypes.byref(o), ctypes.byref(h)),
@@ -337,12 +348,13 @@ class Server:
     poll completions (FP64 norms and server-clock timestamps)."""
 
     def __init__(self, precision: int = PREC_FP64, hbm_budget_bytes: int = 32 << 30,
-                 max_active: int = 512, keep_outputs: bool = False, schedule_kind: int = 0):
+                 max_active: int = 512, keep_outputs: bool = False, schedule_kind: int = 0,
+                 xcd_map: bool = True):
         o = MswServerOptions()
         lib().msw_server_default_options(ctypes.byref(o))
         o.precision, o.hbm_budget_bytes = precision, hbm_budget_bytes
         o.max_active, o.keep_outputs = max_active, int(keep_outputs)
-        o.schedule_kind = schedule_kind
+        o.schedule_kind, o.xcd_map = schedule_kind, int(xcd_map)
         h = ctypes.c_void_p()
         _check(lib().msw_server_create(ctypes.byref(o), ctypes.byref(h)), "msw_server_create")
         self._h = h

@@ -37,6 +37,7 @@ def main():
     ap.add_argument("--stagger", type=int, default=0)
     ap.add_argument("--row-groups", type=int, default=1)
     ap.add_argument("--stream-target", type=int, default=128)
+    ap.add_argument("--xcd-map", type=int, default=1)
     ap.add_argument("--autotune", action="store_true",
                     help="autotune the batch first (needs MSW_BATCH_AUTOTUNE=1)")
     args = ap.parse_args()
@@ -95,7 +96,8 @@ def main():
 
     def batch_of(precision, tune, configured=False):
         kw = dict(schedule_kind=args.schedule_kind, stagger=bool(args.stagger),
-                  row_groups=args.row_groups, stream_target=args.stream_target) if configured else {}
+                  row_groups=args.row_groups, stream_target=args.stream_target,
+                  xcd_map=bool(args.xcd_map)) if configured else {}
         b = msw.Batch(dims, precision=precision, **kw)
         for i, d in enumerate(dims):
             a, m = mats[(d.nm, d.na)]

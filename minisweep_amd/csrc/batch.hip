@@ -226,7 +226,7 @@ long long tick_groups(const TickWork& w) {
     return -1;
   }
   const long long ng = (long long)w.s->cnt[w.launch] * ((w.g->d.ne + w.epw - 1) / w.epw);
-  if (ng >= (1ll << (31 - kEpwBits))) {
+  if (ng >= (1ll << (30 - kEpwBits))) {
     set_error(MSW_ERR_UNSUPPORTED, "internal: tick segment of too many workgroups");
     return -1;
   }
@@ -238,7 +238,7 @@ long long tick_groups(const TickWork& w) {
 // negated MSW_ERR_* code when the assembly checks fail (nothing of the
 // offending launch is enqueued).
 int enqueue_tick(int precision, std::vector<TickWork>& work, const SweepParams* d_params,
-                 hipStream_t st) {
+                 hipStream_t st, bool xcd_map) {
   choose_tick_epw(work, precision);
   std::map<std::tuple<int, int, int, int>, std::vector<const TickWork*>> by_variant;
   for (const TickWork& w : work) by_variant[w.g->variant()].push_back(&w);
@@ -253,7 +253,7 @@ int enqueue_tick(int precision, std::vector<TickWork>& work, const SweepParams* 
         const TickWork& w = *ws[k];
         const long long ng = tick_groups(w);
         if (ng < 0) return -MSW_ERR_INVALID;
-        const long long nwg = xcd_round_ids(ng, w.g->d.nu);
+        const long long nwg = xcd_map ? xcd_round_ids(ng, w.g->d.nu) : ng * w.g->d.nu;
         if (ml.nseg >= kMaxSegments || w.slot < 0) {
           set_error(MSW_ERR_INVALID, "internal: segment table overflow");
           return -MSW_ERR_INVALID;
@@ -261,7 +261,7 @@ int enqueue_tick(int precision, std::vector<TickWork>& work, const SweepParams* 
         ml.start[ml.nseg] = (int)grid;
         ml.prob[ml.nseg] = w.slot;
         ml.item_base[ml.nseg] = w.s->off[w.launch];
-        ml.epw_ng[ml.nseg] = w.epw | (int)ng << kEpwBits;
+        ml.epw_ng[ml.nseg] = w.epw | (xcd_map ? kXcdFlag : 0) | (int)ng << (kEpwBits + 1);
         ml.nseg++;
         grid += nwg;
       }
@@ -282,16 +282,16 @@ int enqueue_tick(int precision, std::vector<TickWork>& work, const SweepParams* 
 // (round-robin over ncs streams), forked from st and joined back to it.
 int enqueue_tick_forked(int precision, std::vector<TickWork>& work, const SweepParams* d_params,
                         hipStream_t st, hipStream_t* cs, hipEvent_t fork, hipEvent_t* join,
-                        int ncs) {
+                        int ncs, bool xcd_map) {
   std::map<std::tuple<int, int, int, int>, std::vector<TickWork>> by_variant;
   for (const TickWork& w : work) by_variant[w.g->variant()].push_back(w);
   const int n = std::min<int>((int)by_variant.size(), ncs);
-  if (n < 2) return enqueue_tick(precision, work, d_params, st);
+  if (n < 2) return enqueue_tick(precision, work, d_params, st, xcd_map);
   (void)hipEventRecord(fork, st);
   for (int c = 0; c < n; ++c) (void)hipStreamWaitEvent(cs[c], fork, 0);
   int nl = 0, c = 0, err = 0;
   for (auto& kv : by_variant) {
-    const int k = enqueue_tick(precision, kv.second, d_params, cs[c++ % n]);
+    const int k = enqueue_tick(precision, kv.second, d_params, cs[c++ % n], xcd_map);
     if (k < 0) err = k;
     else nl += k;
   }
@@ -461,7 +461,7 @@ int batch_enqueue(MswBatch* b, hipStream_t st) {
         if (l >= 0 && l < p.s->nlaunch()) work.push_back(TickWork{i, &p.g, p.s, l});
       }
       if (work.empty()) continue;
-      const int k = enqueue_tick(prec, work, b->d_params, cs);
+      const int k = enqueue_tick(prec, work, b->d_params, cs, b->opt.xcd_map != 0);
       if (k < 0) err = -k;
       else nk += k;
     }
@@ -594,6 +594,7 @@ void msw_batch_default_options(MswBatchOptions* o) {
   o->stagger = 1;
   o->row_groups = 0;
   o->stream_target = 0;
+  o->xcd_map = 1;
 }
 
 int msw_batch_create(int nprob, const MswDims* dims, const MswBatchOptions* opt, MswBatch** out) {
@@ -1112,7 +1113,7 @@ void server_loop(MswServer* s) {
       }
       if (!work.empty()) {
         const int k = enqueue_tick_forked(s->opt.precision, work, s->d_params, s->st,
-                                          s->chain_st, s->fork_ev, s->join_ev, kChainStreams);
+                                          s->chain_st, s->fork_ev, s->join_ev, kChainStreams, s->opt.xcd_map != 0);
         if (k < 0) {  // reported with each request of the tick when it retires
           for (const TickWork& w : work)
             for (Active& a : s->active)
@@ -1168,6 +1169,7 @@ void msw_server_default_options(MswServerOptions* o) {
   o->max_active = 512;
   o->keep_outputs = 0;
   o->schedule_kind = 0;
+  o->xcd_map = 1;
 }
 
 int msw_server_tune_from_batch(MswServer* s, const MswBatch* b) {

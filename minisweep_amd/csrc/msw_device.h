@@ -97,18 +97,29 @@ struct SweepParams {
   int z_off = 0, nz_grid = 0;
   void* fz = nullptr;
   int z_in = 0, z_out = 0;
+  // Outflow faces at the domain's downstream boundary, written in the
+  // reference's Faces layouts with all eight octants when not null
+  // (msw_plan_sweep_faces; whole-grid sweeps): oxy[8][ne][ny][nx][nu][na],
+  // oxz[8][ne][nz][nx][nu][na], oyz[8][ne][nz][ny][nu][na], FP64, scaled by
+  // (1+octant) as the reference stores them.
+  double* oxy = nullptr;
+  double* oxz = nullptr;
+  double* oyz = nullptr;
 };
 
 // A launch that runs segments of several sweeps: segment k covers
 // workgroups [start[k], start[k+1]) (the last one up to the grid size) and is
 // launch-local work of problem prob[k], whose items start at item_base[k].
 // epw_ng[k] packs the energy groups streamed through one workgroup (low
-// kEpwBits bits) and the segment's (item, group chunk) count above them; a
-// segment's workgroup count is that count padded to whole rounds of the XCDs
-// (xcd_round_ids).  Passed by value as a kernel argument.
+// kEpwBits bits), a flag (bit kEpwBits) and the segment's (item, group chunk)
+// count above it.  With the flag the segment's workgroups are that count
+// padded to whole rounds of the XCDs (xcd_round_ids) and mapped to them as
+// a single sweep's are; without it they are count * nu, in id order.
+// Passed by value as a kernel argument.
 constexpr int kMaxSegments = 128;
 constexpr int kEpwBits = 6;
 constexpr int kEpwMask = (1 << kEpwBits) - 1;
+constexpr int kXcdFlag = 1 << kEpwBits;
 struct MultiLaunch {
   int nseg;
   int start[kMaxSegments];

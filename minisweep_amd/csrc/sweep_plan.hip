@@ -810,9 +810,14 @@ int msw_plan_destroy(MswPlan* p) {
 // runs two passes (octants 0-3 upward in z, then 4-7 downward), each over the
 // blocks in its direction: the pass's launches on each block's planes.  The
 // mixed forms first make their operand copies of vi.
-static int sweep_device(MswPlan* p, const double* vi, double* vo, hipStream_t st) {
+static int sweep_device(MswPlan* p, const double* vi, double* vo, hipStream_t st,
+                        double* oxy = nullptr, double* oxz = nullptr,
+                        double* oyz = nullptr) {
   const MswDims& d = p->d;
   SweepParams sp;
+  sp.oxy = oxy;
+  sp.oxz = oxz;
+  sp.oyz = oyz;
   sp.vi = vi;
   sp.vo = vo;
   sp.at = p->at;
@@ -868,6 +873,23 @@ int msw_plan_sweep(MswPlan* p, const double* vi, double* vo,
   hipLaunchKernelGGL(k_pack_matrices, dim3((n + 255) / 256), dim3(256), 0, st,
                      a_from_m, m_from_a, p->at, p->mf, d.nm, d.na, p->na_pad);
   return sweep_device(p, vi, vo, st);
+}
+
+int msw_plan_sweep_faces(MswPlan* p, const double* vi, double* vo, const double* a_from_m,
+                         const double* m_from_a, double* facexy, double* facexz,
+                         double* faceyz, void* stream) {
+  if (!p || !vi || !vo || !a_from_m || !m_from_a || !facexy || !facexz || !faceyz)
+    return set_err(MSW_ERR_INVALID, "null argument to msw_plan_sweep_faces");
+  if (vi == vo) return set_err(MSW_ERR_INVALID, "vi and vo must differ");
+  if (p->engine != MSW_ENGINE_MFMA || p->nblk != 1)
+    return set_err(MSW_ERR_UNSUPPORTED,
+                   "msw_plan_sweep_faces: MFMA engine without z blocks only");
+  hipStream_t st = (hipStream_t)stream;
+  const MswDims& d = p->d;
+  const int n = kNoct * d.nm * d.na;
+  hipLaunchKernelGGL(k_pack_matrices, dim3((n + 255) / 256), dim3(256), 0, st,
+                     a_from_m, m_from_a, p->at, p->mf, d.nm, d.na, p->na_pad);
+  return sweep_device(p, vi, vo, st, facexy, facexz, faceyz);
 }
 
 int msw_plan_sweep_host(MswPlan* p, const double* vi_host, double* vo_host,

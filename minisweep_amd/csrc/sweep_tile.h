@@ -777,6 +777,35 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
           for (int a = 0; a < NAB; ++a) store4(dst + 16 * a, F[a][b]);
         }
     }
+    // the domain's downstream faces in the reference's layouts, on request:
+    // the solved value r = q K (K = 2^exs) times (1+octant), per angle
+    if constexpr (!ZB) {
+      if (p.oyz) {
+        auto put = [&](double* base, const int b) {
+#pragma unroll
+          for (int a = 0; a < NAB; ++a)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int ia = ia0 + 16 * a + PR::row(g, r);
+              if (ia < p.na)
+                base[ia] = ldexpT((double)F[a][b][r], exs[b]) * (double)(1 + oct);
+            }
+        };
+#pragma unroll
+        for (int b = 0; b < kTY; ++b) {
+          if (!val[b]) continue;
+          const int ie = ie0 + es[b];
+          const int zg = dz > 0 ? zs[b] : nz - 1 - zs[b];
+          const size_t oe = (size_t)oct * p.ne + ie;
+          if (!x_out && j == wx - 1)
+            put(p.oyz + (((oe * nz + zg) * ny + yb[b]) * nu + iu) * p.na, b);
+          if (!y_out && kTY * wyi + b == wy - 1)
+            put(p.oxz + (((oe * nz + zg) * nx + x) * nu + iu) * p.na, b);
+          if (zs[b] == nz - 1)
+            put(p.oxy + (((oe * ny + yb[b]) * nx + x) * nu + iu) * p.na, b);
+        }
+      }
+    }
     if (ZB && p.z_out) {  // the block's last plane is the next block's z inflow
 #pragma unroll
       for (int b = 0; b < kTY; ++b)
@@ -926,7 +955,10 @@ __global__ __launch_bounds__(64 * NWA * NY) __attribute__((amdgpu_waves_per_eu(
     if (k + step < ml.nseg && ml.start[k + step] <= blk) k += step;
   const SweepParams p = probs[ml.prob[k]];
   const int epw = ml.epw_ng[k] & kEpwMask;
-  const int w = xcd_unmap(blk - ml.start[k], p.nu, ml.epw_ng[k] >> kEpwBits);
+  const int local = blk - ml.start[k];
+  const int w = ml.epw_ng[k] & kXcdFlag
+                    ? xcd_unmap(local, p.nu, ml.epw_ng[k] >> (kEpwBits + 1))
+                    : local;
   if (w < 0) return;
   sweep_block<PR, NM, NAC, NWA, NY>(p, ml.item_base[k], w, epw);
 }

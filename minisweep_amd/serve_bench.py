@@ -134,6 +134,7 @@ def main():
     ap.add_argument("--stagger", type=int, default=0)
     ap.add_argument("--row-groups", type=int, default=1)
     ap.add_argument("--stream-target", type=int, default=128)
+    ap.add_argument("--xcd-map", type=int, default=1)
     ap.add_argument("--plan-defaults", action="store_true")
     ap.add_argument("--tune", action="store_true",
                     help="autotune on the batched workload first (needs MSW_BATCH_AUTOTUNE=1)")
@@ -148,7 +149,8 @@ def main():
 
     prec = {"fp64": msw.PREC_FP64, "fp32": msw.PREC_MIXED_FP32,
             "bf16": msw.PREC_MIXED_BF16}[args.precision]
-    served = dict(precision=prec, hbm_budget_bytes=int(args.budget_gib * (1 << 30)),
+    served = dict(precision=prec, xcd_map=bool(args.xcd_map),
+                  hbm_budget_bytes=int(args.budget_gib * (1 << 30)),
                   max_active=args.max_active)
     seq = dict(precision=msw.PREC_FP64, hbm_budget_bytes=int(args.budget_gib * (1 << 30)),
                max_active=1)

@@ -42,6 +42,7 @@ def lib():
         L.oracle_sweep.argtypes = [OracleDims, dp, dp, dp, dp, ctypes.c_int]
         L.oracle_sweep_tileoctants.argtypes = [OracleDims, dp, dp, dp, dp, ctypes.c_int]
         L.oracle_sweep_ld.argtypes = [OracleDims, dp, dp, dp, dp, ctypes.c_int]
+        L.oracle_sweep_faces.argtypes = [OracleDims, dp, dp, dp, dp, dp, dp, dp, ctypes.c_int]
         L.oracle_sweep_range.argtypes = [OracleDims, dp, dp, dp, dp,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.oracle_norms.argtypes = [OracleDims, dp, dp, dp, dp]
@@ -107,6 +108,21 @@ def sweep_ld(d: OracleDims, a, m, vi, nthreads: int = 0) -> np.ndarray:
     lib().oracle_sweep_ld(d, _p(np.ascontiguousarray(a)), _p(np.ascontiguousarray(m)),
                           _p(vi), _p(vo), nthreads)
     return vo
+
+
+def sweep_faces(d: OracleDims, a, m, vi, nthreads: int = 0):
+    """(vo, facexy, facexz, faceyz): the sweep and each octant's outflow faces
+    at the downstream boundary, reference layouts with all 8 octants
+    ([8][ne][ny][nx][nu][na], [8][ne][nz][nx][nu][na], [8][ne][nz][ny][nu][na]),
+    scaled by (1+octant) as the reference stores them."""
+    vo = np.empty_like(vi)
+    nx, ny, nz, ne, na, nu = d.ncell_x, d.ncell_y, d.ncell_z, d.ne, d.na, d.nu
+    fxy = np.empty(8 * ne * ny * nx * nu * na)
+    fxz = np.empty(8 * ne * nz * nx * nu * na)
+    fyz = np.empty(8 * ne * nz * ny * nu * na)
+    lib().oracle_sweep_faces(d, _p(a), _p(m), _p(np.ascontiguousarray(vi)), _p(vo), _p(fxy),
+                             _p(fxz), _p(fyz), nthreads)
+    return vo, fxy, fxz, fyz
 
 
 def sweep_range(d: OracleDims, a, m, vi, vo, ie_begin, ie_end, nthreads=0):

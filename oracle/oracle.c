@@ -256,9 +256,33 @@ static void sweep_one(const OracleDims* d, const double* A, const double* M,
             d->ncell_y, 0, d->ncell_z);
 }
 
+static void sweep_range_faces(OracleDims d, const double* A, const double* M,
+                              const double* vi, double* vo, int ie_begin, int ie_end,
+                              int nthreads, double* out_xy, double* out_xz, double* out_yz);
+
 void oracle_sweep_range(OracleDims d, const double* A, const double* M,
                         const double* vi, double* vo, int ie_begin, int ie_end,
                         int nthreads) {
+  sweep_range_faces(d, A, M, vi, vo, ie_begin, ie_end, nthreads, NULL, NULL, NULL);
+}
+
+/* The sweep, and each octant's outflow faces at the domain's downstream
+ * boundary in the reference's Faces layouts with all eight octants
+ * (array_accessors_kernels.h:481-486, 545-550, 609-614):
+ *   out_xy[oct][ie][iy][ix][iu][ia], out_xz[oct][ie][iz][ix][iu][ia],
+ *   out_yz[oct][ie][iz][iy][iu][ia].
+ * After an octant's sweep the face arrays hold the last cell's value of each
+ * line, scaled by (1+octant) as the reference stores it
+ * (quantities_testing_kernels.h:437-443). */
+void oracle_sweep_faces(OracleDims d, const double* A, const double* M, const double* vi,
+                        double* vo, double* out_xy, double* out_xz, double* out_yz,
+                        int nthreads) {
+  sweep_range_faces(d, A, M, vi, vo, 0, d.ne, nthreads, out_xy, out_xz, out_yz);
+}
+
+static void sweep_range_faces(OracleDims d, const double* A, const double* M,
+                              const double* vi, double* vo, int ie_begin, int ie_end,
+                              int nthreads, double* out_xy, double* out_xz, double* out_yz) {
   const int nx = d.ncell_x, ny = d.ncell_y, nz = d.ncell_z;
   const size_t nfxy = (size_t)nx * ny * d.na * d.nu;
   const size_t nfxz = (size_t)nx * nz * d.na * d.nu;
@@ -285,8 +309,15 @@ void oracle_sweep_range(OracleDims d, const double* A, const double* M,
     double* fyz = (double*)malloc(sizeof(double) * nfyz);
     double* vs = (double*)malloc(sizeof(double) * (size_t)d.na * d.nu);
     int octant;
-    for (octant = 0; octant < NOCT; ++octant)
+    for (octant = 0; octant < NOCT; ++octant) {
       sweep_one(&d, A, M, vi, vo, octant, ie, fxy, fxz, fyz, vs);
+      if (out_xy) {
+        const size_t k = (size_t)octant * d.ne + ie;
+        memcpy(out_xy + k * nfxy, fxy, sizeof(double) * nfxy);
+        memcpy(out_xz + k * nfxz, fxz, sizeof(double) * nfxz);
+        memcpy(out_yz + k * nfyz, fyz, sizeof(double) * nfyz);
+      }
+    }
     free(fxy);
     free(fxz);
     free(fyz);

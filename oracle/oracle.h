@@ -62,6 +62,16 @@ void oracle_sweep_tileoctants(OracleDims d, const double* a_from_m, const double
 void oracle_sweep_ld(OracleDims d, const double* a_from_m, const double* m_from_a,
                      const double* vi, double* vo, int nthreads);
 
+/* The sweep and each octant's outflow faces at the domain's downstream
+ * boundary, in the reference's Faces layouts with all eight octants
+ * (array_accessors_kernels.h:481-486, 545-550, 609-614):
+ * out_xy[8][ne][ny][nx][nu][na], out_xz[8][ne][nz][nx][nu][na],
+ * out_yz[8][ne][nz][ny][nu][na]; values scaled by (1+octant) as the
+ * reference stores them (quantities_testing_kernels.h:437-443). */
+void oracle_sweep_faces(OracleDims d, const double* a_from_m, const double* m_from_a,
+                        const double* vi, double* vo, double* out_xy, double* out_xz,
+                        double* out_yz, int nthreads);
+
 /* Same sweep restricted to the energy groups [ie_begin, ie_end): used to time
  * a bounded sample of a large problem. */
 void oracle_sweep_range(OracleDims d, const double* a_from_m,

@@ -34,6 +34,7 @@ static int run_batch(std::vector<MswDims> dims, int prec, int graph, unsigned se
   o.precision = prec; o.use_graph = graph; o.stagger = seed & 1; o.schedule_kind = (seed >> 1) & 1;
   const int rgs[4] = {0, 1, 2, 4};
   o.row_groups = rgs[(seed >> 2) % 4]; o.stream_target = seed % 3 == 0 ? 32 : 0;
+  o.xcd_map = (seed >> 4) & 1 ? 0 : 1;
   MswBatch* b = nullptr;
   int rc = msw_batch_create((int)dims.size(), dims.data(), &o, &b);
   if (rc) { printf("create rc %d: %s\n", rc, msw_last_error()); return rc; }

@@ -157,13 +157,15 @@ void launch_mfma_multi(int precision, int nm, int nac, int nwa, int ny, unsigned
       check_range(probs + m.prob[k], sizeof(SweepParams), "probs[prob[k]]");
       const SweepParams p = probs[m.prob[k]];
       const int epw = m.epw_ng[k] & kEpwMask;
-      const int ng = m.epw_ng[k] >> kEpwBits;
+      const bool xmap = (m.epw_ng[k] & kXcdFlag) != 0;
+      const int ng = m.epw_ng[k] >> (kEpwBits + 1);
       if (epw < 1 || epw > kMaxStreamGroups) DIE("epw %d", epw);
       const long long seg_end = k + 1 < m.nseg ? m.start[k + 1] : (long long)grid;
-      if (seg_end - m.start[k] != xcd_round_ids(ng, p.nu)) DIE("segment %d: %lld workgroups for %d groups", k, seg_end - m.start[k], ng);
-      if (m.start[k] % 8) DIE("segment %d starts off an XCD round", k);
+      const long long want = xmap ? xcd_round_ids(ng, p.nu) : (long long)ng * p.nu;
+      if (seg_end - m.start[k] != want) DIE("segment %d: %lld workgroups for %d groups", k, seg_end - m.start[k], ng);
+      if (xmap && m.start[k] % 8) DIE("segment %d starts off an XCD round", k);
       int bid = (int)blk - m.start[k];
-      {  // the kernel's XCD mapping: id -> (group, unknown), ids past the groups exit
+      if (xmap) {  // the kernel's XCD mapping: id -> (group, unknown), ids past the groups exit
         const int kk = bid / 8, g = (kk / p.nu) * 8 + bid % 8;
         if (g >= ng) { ++g_checked_wg; continue; }
         bid = g * p.nu + kk % p.nu;

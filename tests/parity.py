@@ -96,3 +96,35 @@ TESTER_SHAPES = [
     (5, 5, 5, 30, 33, 4),   # sweep defaults, scripts' NM=4
     (8, 8, 8, 4, 32, 16),   # 8^3 ne4 na32 nm16 CPU configuration
 ]
+
+
+def face_closed_form(shape):
+    """Outflow faces of the manufactured problem: every cell's solved value
+    is (1+a) 2^(a&7) S(cell) E(ie) U(iu) (the boundary formula evaluated at
+    the cell, quantities_testing_kernels.h:219-332 with the solve :380-446),
+    and a face holds the last cell of its line times (1+octant)."""
+    ncx, ncy, ncz, ne, nm, na, nu = shape
+    ia = np.arange(na)
+    q = ((1 + ia) * (1 << (ia & 7))).astype(np.float64)
+    e = np.array([sf_energy(i) for i in range(ne)], dtype=np.float64)
+    u = np.array([sf_unknown(i) for i in range(nu)], dtype=np.float64)
+
+    def s(x, y, z):
+        return np.float64(1 << ((x + 3 * y + 7 * z + 2) & 3))
+    fxy = np.empty((8, ne, ncy, ncx, nu, na))
+    fxz = np.empty((8, ne, ncz, ncx, nu, na))
+    fyz = np.empty((8, ne, ncz, ncy, nu, na))
+    eu = e[:, None, None] * u[None, :, None] * q[None, None, :]
+    for o in range(8):
+        xl = 0 if o & 1 else ncx - 1
+        yl = 0 if o & 2 else ncy - 1
+        zl = 0 if o & 4 else ncz - 1
+        for y in range(ncy):
+            for x in range(ncx):
+                fxy[o, :, y, x] = eu * s(x, y, zl) * (1 + o)
+        for z in range(ncz):
+            for x in range(ncx):
+                fxz[o, :, z, x] = eu * s(x, yl, z) * (1 + o)
+            for y in range(ncy):
+                fyz[o, :, z, y] = eu * s(xl, y, z) * (1 + o)
+    return fxy.ravel(), fxz.ravel(), fyz.ravel()

@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 import oracle
-from parity import TESTER_SHAPES, assert_close, closed_form_normsq, NORM_TOL
+from parity import TESTER_SHAPES, assert_close, closed_form_normsq, face_closed_form, NORM_TOL
 
 GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
 
@@ -275,3 +275,18 @@ def test_mixed_model_error_levels():
     assert abs(float(np.sum(v32 * v32)) - n_ref) / n_ref <= 1e-5
     v16 = mixed_model.sweep(shape, a, m, vi, "bf16")
     assert np.max(np.abs(v16 - ref)) / np.max(np.abs(ref)) > 0.1
+
+
+@pytest.mark.parametrize("shape", [(5, 4, 6, 2, 16, 64, 2), (3, 7, 2, 1, 4, 13, 3)])
+def test_oracle_faces_manufactured_closed_form(shape):
+    """The oracle's outflow faces (the reference's Faces layouts) on the
+    manufactured input equal the closed form exactly; its vo is exact too."""
+    d = oracle.dims(*shape)
+    a, m = oracle.init_matrices(d)
+    vi = oracle.init_state(d)
+    vo, fxy, fxz, fyz = oracle.sweep_faces(d, a, m, vi)
+    np.testing.assert_array_equal(vo, vi)
+    cxy, cxz, cyz = face_closed_form(shape)
+    np.testing.assert_array_equal(fxy, cxy)
+    np.testing.assert_array_equal(fxz, cxz)
+    np.testing.assert_array_equal(fyz, cyz)
