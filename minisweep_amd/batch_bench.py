@@ -38,6 +38,8 @@ def main():
     ap.add_argument("--row-groups", type=int, default=1)
     ap.add_argument("--stream-target", type=int, default=128)
     ap.add_argument("--xcd-map", type=int, default=1)
+    ap.add_argument("--skip-baselines", action="store_true",
+                    help="time only the configured batch (no sequential or FP64 batch runs)")
     ap.add_argument("--autotune", action="store_true",
                     help="autotune the batch first (needs MSW_BATCH_AUTOTUNE=1)")
     args = ap.parse_args()
@@ -89,8 +91,10 @@ def main():
             a, m = mats[(d.nm, d.na)]
             p.sweep(vi, vo, a, m)
 
-    t_seq = time_runs(seq, max(1, args.reps // 2))
-    bad_seq = check()
+    t_seq, bad_seq = float("nan"), 0
+    if not args.skip_baselines:
+        t_seq = time_runs(seq, max(1, args.reps // 2))
+        bad_seq = check()
     for p in plans:
         p.close()
 
@@ -109,10 +113,12 @@ def main():
             t_tune = time.perf_counter() - t0
         return b, t_tune
 
-    b64, _ = batch_of(msw.PREC_FP64, False)
-    t_b64 = time_runs(b64.run, args.reps)
-    bad_b64 = check()
-    b64.close()
+    t_b64, bad_b64 = float("nan"), 0
+    if not args.skip_baselines:
+        b64, _ = batch_of(msw.PREC_FP64, False)
+        t_b64 = time_runs(b64.run, args.reps)
+        bad_b64 = check()
+        b64.close()
 
     bm, t_tune = batch_of(prec, args.autotune, configured=True)
     t_bm = time_runs(bm.run, args.reps)
