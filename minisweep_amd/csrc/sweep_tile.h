@@ -279,9 +279,11 @@ template <class PR, int NM, int NAC, int NWA, int NY>
 struct Occupancy {
   static constexpr int bytes = (int)sizeof(typename PR::T) * NAC;
   static constexpr bool rows_ok = NY == 1 || (NWA == 2 && NY == 2 && NM < 16);
+  // FP32 form, 8 moments, one angle wave and one row group: 2 registers past 256
+  static constexpr bool split_ok = !PR::kSplit || NM < 8 || NWA > 1 || NY > 1;
   static constexpr int value =
       bytes <= 64                                                          ? 2
-      : bytes <= 128 && sizeof(typename PR::T) == 4 && rows_ok &&
+      : bytes <= 128 && sizeof(typename PR::T) == 4 && rows_ok && split_ok &&
               (NM < 16 || PR::kPacked)                                     ? 2
       : bytes <= 128 && NWA >= 2 && NY == 1 && NM < 16                     ? 2
                                                                            : 1;
