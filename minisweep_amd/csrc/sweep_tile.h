@@ -285,7 +285,8 @@ struct Occupancy {
       bytes <= 64                                                          ? 2
       : bytes <= 128 && sizeof(typename PR::T) == 4 && rows_ok && split_ok &&
               (NM < 16 || PR::kPacked)                                     ? 2
-      : bytes <= 128 && NWA >= 2 && NY == 1 && NM < 16                     ? 2
+      : bytes <= 128 && NWA >= 2 && NY == 1 && (NM < 16 || sizeof(typename PR::T) == 8)
+                                                                           ? 2
                                                                            : 1;
   // the many-problem kernel needs a few registers more (problem lookup): two
   // waves only at 16 angles per wave in FP32 / BF16
