@@ -83,6 +83,16 @@ typedef struct {
                                remaining visits first; 1 the reference's KBA octant
                                order {0,4,2,6,3,7,1,5}
                                (src/3_sweeper/stepscheduler_kba.c:155) */
+  int energy_offset;        /* global index of the plan's first energy group (0): the
+                               plan sweeps groups [energy_offset, energy_offset + ne)
+                               of a larger problem, whose per-group scale factors
+                               (src/3_sweeper/quantities_testing_kernels.h:52-70 of the
+                               reference: they depend on the global group index)
+                               it applies.  Energy groups do not couple, so the
+                               groups of one problem can be split over plans (one
+                               GPU's share each) and their results are those of the
+                               whole problem's groups; state arrays hold the plan's
+                               ne groups in the usual layout */
 } MswPlanOptions;
 
 void msw_plan_default_options(MswPlanOptions* opt);

@@ -40,7 +40,7 @@ class MswPlanOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("engine", ctypes.c_int),
                 ("angles_per_wave", ctypes.c_int), ("groups_per_workgroup", ctypes.c_int),
                 ("tile_rows", ctypes.c_int), ("z_block", ctypes.c_int),
-                ("schedule_kind", ctypes.c_int)]
+                ("schedule_kind", ctypes.c_int), ("energy_offset", ctypes.c_int)]
 
 
 class MswBatchOptions(ctypes.Structure):
@@ -182,7 +182,7 @@ class Plan:
     def __init__(self, d: MswDims, precision: int = PREC_FP64,
                  engine: Optional[int] = None, angles_per_wave: int = 0,
                  groups_per_workgroup: int = 0, tile_rows: int = 0, z_block: int = 0,
-                 schedule_kind: int = 0):
+                 schedule_kind: int = 0, energy_offset: int = 0):
         self.d = d
         self.precision = precision
         o = MswPlanOptions()
@@ -195,6 +195,7 @@ class Plan:
         o.tile_rows = tile_rows
         o.z_block = z_block
         o.schedule_kind = schedule_kind
+        o.energy_offset = energy_offset
         self.engine = o.engine
         h = ctypes.c_void_p()
         _check(lib().msw_plan_create_ex(ctypes.byref(d), ctypes.byref(o), ctypes.byref(h)),

@@ -157,7 +157,8 @@ float* vhi_of(const Geom& g, float* vo32, int precision) {
 // The mixed forms' operand copies of one problem's vi (nothing in FP64).
 void enqueue_operand_copy(const Geom& g, const double* vi, float* vo32, int precision,
                           hipStream_t st) {
-  if (vo32) launch_operand_copy(precision, g.d, vi, vhi_of(g, vo32, precision), vlo_of(g, vo32), st);
+  if (vo32)
+    launch_operand_copy(precision, g.d, vi, vhi_of(g, vo32, precision), vlo_of(g, vo32), 0, st);
 }
 
 SweepParams make_params(const Geom& g, const double* vi, double* vo, const double* at,

@@ -87,6 +87,7 @@ struct SweepParams {
   int nx, ny, nz, ne, na, nu;
   int na_pad;  // na rounded up to whole angle chunks (matrix/face stride)
   int ntx, nty;
+  int e_off = 0;  // global index of group 0 (scale factors use e_off + ie)
   // z blocks (MFMA engine, FP64): the sweep runs one block of nz planes at a
   // time, vi/vo pointing at the block.  z_off is the block's first plane in
   // the grid and nz_grid the grid's extent (0: not blocked, nz is the grid).

@@ -59,11 +59,12 @@ MSW_DECLARE_LAUNCH(BF16, 16)
 #undef MSW_DECLARE_LAUNCH
 // The mixed forms' m2a operand copies of vi (each value divided by its
 // cell's scale K = S E U): FP32 form hi (FP32) and lo (BF16 of the rest), BF16
-// form lo = the BF16 value (hi unused).  Bytes of device memory they take
-// for a state of n values (0 in FP64).
+// form lo = the BF16 value (hi unused); e_off is the global index of the
+// state's group 0.  Bytes of device memory they take for a state of n values
+// (0 in FP64).
 size_t operand_copy_bytes(int precision, size_t n);
 void launch_operand_copy(int precision, const MswDims& d, const double* vi, float* hi,
-                         unsigned short* lo, hipStream_t st);
+                         unsigned short* lo, int e_off, hipStream_t st);
 // Bytes of one face value of the MFMA kernels in a precision.
 size_t mfma_face_elem_bytes(int precision);
 // MFMA tile: mfma_tile_x() cells along x, ny row groups of mfma_tile_rows()

@@ -25,7 +25,8 @@ namespace {
 __global__ __launch_bounds__(256) void k_operand_copy(const double* __restrict__ vi,
                                                       float* __restrict__ hi,
                                                       unsigned short* __restrict__ lo, size_t n,
-                                                      int nx, int ny, int ne, int nm, int nu) {
+                                                      int nx, int ny, int ne, int nm, int nu,
+                                                      int e_off) {
   for (size_t i = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) * 4; i < n;
        i += (size_t)gridDim.x * blockDim.x * 4) {
     unsigned c = (unsigned)(i / (size_t)nm);  // (z, group, y, x, unknown) of the run
@@ -37,7 +38,7 @@ __global__ __launch_bounds__(256) void k_operand_copy(const double* __restrict__
     c /= (unsigned)ny;
     const int ie = (int)(c % (unsigned)ne);
     const int z = (int)(c / (unsigned)ne);
-    const int e = -(sf_space_exp(x, y, z) + sf_energy_exp(ie) + sf_unknown_exp(u));
+    const int e = -(sf_space_exp(x, y, z) + sf_energy_exp(e_off + ie) + sf_unknown_exp(u));
     const double2 a = *reinterpret_cast<const double2*>(vi + i);
     const double2 b = *reinterpret_cast<const double2*>(vi + i + 2);
     const double v[4] = {__builtin_ldexp(a.x, e), __builtin_ldexp(a.y, e),
@@ -99,13 +100,13 @@ size_t operand_copy_bytes(int precision, size_t n) {
 }
 
 void launch_operand_copy(int precision, const MswDims& d, const double* vi, float* hi,
-                         unsigned short* lo, hipStream_t st) {
+                         unsigned short* lo, int e_off, hipStream_t st) {
   if (precision != MSW_PREC_MIXED_FP32 && precision != MSW_PREC_MIXED_BF16) return;
   const size_t n = (size_t)d.ncell_x * d.ncell_y * d.ncell_z * d.ne * d.nm * d.nu;
   const unsigned blocks = (unsigned)std::min<size_t>((n / 4 + 255) / 256, 8192);
   hipLaunchKernelGGL(k_operand_copy, dim3(blocks), dim3(256), 0, st, vi,
                      precision == MSW_PREC_MIXED_FP32 ? hi : nullptr, lo, n, d.ncell_x,
-                     d.ncell_y, d.ne, d.nm, d.nu);
+                     d.ncell_y, d.ne, d.nm, d.nu, e_off);
 }
 
 

@@ -126,7 +126,7 @@ __global__ __launch_bounds__(512) void k_sweep_fp64(SweepParams p,
 
   const double oct_s = (double)(1 + oct);
   const double oct_r = 1.0 / oct_s;
-  const double eu_oct = (double)sf_energy(ie) * (double)sf_unknown(iu) * oct_s;
+  const double eu_oct = (double)sf_energy(p.e_off + ie) * (double)sf_unknown(iu) * oct_s;
 
   const size_t face_ouy = (((size_t)oct * p.ne + ie) * p.nu + iu);
   double* __restrict__ fyz = (double*)p.fyz + face_ouy * ny * nz * nap;  // [y][z][ia]
@@ -435,6 +435,7 @@ struct MswPlan {
   int engine;  // MSW_ENGINE_*
   int nac, nwa, na_pad, ntx, nty;
   int ny;  // row groups per tile (MFMA engine)
+  int e_off = 0;  // global index of the plan's first energy group
   std::vector<int> launch_epw;  // energy groups per workgroup of each launch (MFMA)
   size_t lds_bytes;
   double* at;  // a_from_m transposed [8][na][nm] (+ padding)
@@ -562,6 +563,7 @@ void msw_plan_default_options(MswPlanOptions* o) {
   o->tile_rows = 0;
   o->z_block = 0;
   o->schedule_kind = 0;
+  o->energy_offset = 0;
 }
 
 int msw_plan_create(const MswDims* d, int precision, MswPlan** out) {
@@ -638,6 +640,11 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
     delete p;
     return set_error(MSW_ERR_INVALID, "z_block must be >= 0");
   }
+  if (opt->energy_offset < 0) {
+    delete p;
+    return set_err(MSW_ERR_INVALID, "energy_offset must be >= 0 (got %d)", opt->energy_offset);
+  }
+  p->e_off = opt->energy_offset;
   if (opt->schedule_kind != 0 && opt->schedule_kind != 1) {
     delete p;
     return set_err(MSW_ERR_INVALID, "schedule_kind must be 0 or 1 (got %d)", opt->schedule_kind);
@@ -838,8 +845,9 @@ static int sweep_device(MswPlan* p, const double* vi, double* vo, hipStream_t st
   sp.na_pad = p->na_pad;
   sp.ntx = p->ntx;
   sp.nty = p->nty;
+  sp.e_off = p->e_off;
   if (p->nblk == 1) {
-    if (p->vlo) launch_operand_copy(p->precision, d, vi, p->vhi, p->vlo, st);
+    if (p->vlo) launch_operand_copy(p->precision, d, vi, p->vhi, p->vlo, p->e_off, st);
     return launch_range(p, sp, st, 0, p->launch_off.size());
   }
   const size_t slab = msw_size_state(&d) / p->nblk;
@@ -974,6 +982,7 @@ int msw_plan_sweep_host(MswPlan* p, const double* vi_host, double* vo_host,
       rc = sweep_device(p, p->slab_vi[sl], p->slab_vo[sl], st);
     } else {
       SweepParams b;
+      b.e_off = p->e_off;
       b.vi = p->slab_vi[sl];
       b.vo = p->slab_vo[sl];
       b.at = p->at;

@@ -432,7 +432,8 @@ __device__ __forceinline__ void sweep_tile(const SweepParams& p, const TileItem 
   // groups streamed, 2 bits each; + the unknown's: the scale K of a cell is
   // 2^(space + group + unknown exponent)
   unsigned long long emask = 0;
-  for (int k = 0; k < nge; ++k) emask |= (unsigned long long)sf_energy_exp(ie0 + k) << (2 * k);
+  for (int k = 0; k < nge; ++k)
+    emask |= (unsigned long long)sf_energy_exp(p.e_off + ie0 + k) << (2 * k);
   const int su = sf_unknown_exp(iu);
 
   T F[NAB][kTY][4];  // outgoing face value (= solved value q) of the lane's cells

@@ -194,7 +194,7 @@ void launch_pack(const double* afm, const double* mfa, double* at, double* mf, i
   enqueue([=] { check_range(at, (size_t)8 * na_pad * nm * 8, "at out@run"); check_range(afm, (size_t)8 * nm * na * 8, "afm@run"); });
 }
 void launch_operand_copy(int precision, const MswDims& d, const double* vi, float* hi,
-                         unsigned short* lo, hipStream_t) {
+                         unsigned short* lo, int, hipStream_t) {
   const size_t n = (size_t)d.ncell_x * d.ncell_y * d.ncell_z * d.ne * d.nm * d.nu;
   enqueue([=] {
     check_range(vi, n * 8, "copy vi");
