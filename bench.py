@@ -26,6 +26,11 @@ Prints ONE JSON line.  `value` is the FP64 sweep's GFLOP/s.  Also in it:
                oracle alone, with the reason in "reference_missing".
   mixed_fp64_last  the mixed FP32 form with a final FP64 sweep: of four
                iterations the first three in FP32, the last in FP64.
+
+With --shards G --shard R it times instead the share of the energy groups
+that rank R of G would sweep on its own GPU (a plan with energy_offset, the
+whole problem's manufactured state cut to those groups) and prints that one
+line.
 """
 from __future__ import annotations
 
@@ -176,6 +181,71 @@ def time_precision(msw, torch, d, prec_name, args, ta, tm, vi, vo, flops_sweep):
     }
 
 
+def share_bench(args):
+    """One rank's share of the energy groups (--shards G --shard R): the
+    groups that rank R of G would sweep on its GPU (minisweep_amd/shard.py),
+    swept by a plan whose energy_offset is the share's first group, from the
+    whole problem's manufactured state cut to those groups.  Prints one JSON
+    line; the G ranks would run such a sweep each, in parallel, plus one
+    all-reduce of two values."""
+    import torch
+    import minisweep_amd as msw
+    from minisweep_amd import shard
+
+    dev = torch.device("cuda:0")
+    n = args.ncell
+    d = msw.dims(n, n, n, args.ne, args.nm, args.na, args.nu)
+    ds, off = shard.share_dims(d, args.shards, args.shard)
+    whole = torch.empty(msw.size_state(d), dtype=torch.float64, device=dev)
+    msw.init_state(d, whole)
+    vi0 = shard.take_groups(whole, d, off, off + ds.ne).reshape(-1)
+    del whole
+    vi, vo = vi0.clone(), torch.empty_like(vi0)
+    a, m = msw.init_matrices(d.nm, d.na)
+    ta, tm = torch.from_numpy(a).to(dev), torch.from_numpy(m).to(dev)
+    name = args.precision or "fp64"
+    prec = {"fp64": msw.PREC_FP64, "fp32": msw.PREC_MIXED_FP32,
+            "bf16": msw.PREC_MIXED_BF16}[name]
+    plan = msw.Plan(ds, prec, angles_per_wave=args.angles_per_wave,
+                    groups_per_workgroup=args.groups_per_workgroup, tile_rows=args.tile_rows,
+                    energy_offset=off)
+    stream = torch.cuda.current_stream()
+    bufs = [vi, vo]
+    for i in range(args.warmup):
+        plan.sweep(bufs[i % 2], bufs[(i + 1) % 2], ta, tm, stream)
+    vi.copy_(vi0)
+    plan.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        plan.sweep(bufs[i % 2], bufs[(i + 1) % 2], ta, tm, stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = sum(plan.last_timings_ms())
+    plan.set_timing(False)
+    exact = bool(torch.equal(bufs[args.steps % 2], vi0))  # S(vi) = vi, manufactured
+    plan.close()
+    flops = msw.flops(ds, 1)
+    print(json.dumps({
+        "metric": "one GPU's share of the sweep (energy groups split over ranks), GFLOP/s",
+        "value": round(flops * args.steps / wall / 1e9, 3),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "dtype": {"fp64": "f64", "fp32": "f32+bf16 (mixed)", "bf16": "bf16 (mixed)"}[name],
+        "data": "synthetic: the whole problem's manufactured state, this share's groups",
+        "config": {"workload": (f"share {args.shard} of {args.shards}: energy groups "
+                                f"[{off}, {off + ds.ne}) of the {n}^3 ne{args.ne} "
+                                f"na{args.na} nm{args.nm} nu{args.nu} sweep, {name}")},
+        "share": {"ranks": args.shards, "rank": args.shard, "groups": [off, off + ds.ne]},
+        "kernel_ms_per_sweep": round(kernel_ms, 3),
+        "exact": exact,
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,6 +269,9 @@ def main():
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="after the sweeps, one device-to-device copy of this many "
                          "GiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--shards", type=int, default=1,
+                    help="time one rank's share of the energy groups, of this many ranks")
+    ap.add_argument("--shard", type=int, default=0, help="the rank whose share --shards times")
     ap.add_argument("--launch-times", action="store_true",
                     help="print the last timed sweep's per-launch ms to stderr")
     args = ap.parse_args()
@@ -208,6 +281,9 @@ def main():
               "supported (the sweep would shard by energy group across GPUs; "
               "only the one-GPU share is built)", file=sys.stderr)
         sys.exit(2)
+    if args.shards > 1:
+        share_bench(args)
+        return
 
     # the reference's CPU runs go first, before this process touches the GPU:
     # its OpenMP-threads build on the host's cores (the reported baseline) and
