@@ -38,6 +38,9 @@ def main():
     ap.add_argument("--row-groups", type=int, default=1)
     ap.add_argument("--stream-target", type=int, default=128)
     ap.add_argument("--xcd-map", type=int, default=1)
+    ap.add_argument("--class-nac", action="append", default=[], metavar="NM,NA,NAC",
+                    help="angles per wave for the problems with these nm, na in the "
+                         "configured batch (repeatable)")
     ap.add_argument("--skip-baselines", action="store_true",
                     help="time only the configured batch (no sequential or FP64 batch runs)")
     ap.add_argument("--autotune", action="store_true",
@@ -103,6 +106,10 @@ def main():
                   row_groups=args.row_groups, stream_target=args.stream_target,
                   xcd_map=bool(args.xcd_map)) if configured else {}
         b = msw.Batch(dims, precision=precision, **kw)
+        if configured:
+            for spec in args.class_nac:
+                nm, na, nac = (int(x) for x in spec.split(","))
+                b.set_class_nac(nm, na, nac)
         for i, d in enumerate(dims):
             a, m = mats[(d.nm, d.na)]
             b.set_problem(i, vis[i], vos[i], a, m)
