@@ -20,25 +20,33 @@ namespace {
 //              leaves (together vi to about 2^-33 relative);
 //   split = 0: bf = its BF16 rounding through FP32 (the two roundings the
 //              BF16 form's contraction applies to its operand).
-// A thread handles a run of 4 consecutive values: 4 moments of one (cell,
-// unknown), nm being a multiple of 4.
+// A thread handles runs of 4 consecutive values: 4 moments of one (cell,
+// unknown), nm being a power of two from 4 (2^nm_log2).  Block b copies part
+// `b % chunks` of the (z, group) plane b / chunks, so that the plane's z and
+// group (and the group's scale) are fixed per block and a run needs only its
+// cell's index within the plane.  I: the in-plane index type, 32 bits
+// unless a plane holds 2^32 values or more.
+template <class I>
 __global__ __launch_bounds__(256) void k_operand_copy(const double* __restrict__ vi,
                                                       float* __restrict__ hi,
-                                                      unsigned short* __restrict__ lo, size_t n,
-                                                      int nx, int ny, int ne, int nm, int nu,
-                                                      int e_off) {
-  for (size_t i = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) * 4; i < n;
-       i += (size_t)gridDim.x * blockDim.x * 4) {
-    unsigned c = (unsigned)(i / (size_t)nm);  // (z, group, y, x, unknown) of the run
-    const int u = (int)(c % (unsigned)nu);
-    c /= (unsigned)nu;
-    const int x = (int)(c % (unsigned)nx);
-    c /= (unsigned)nx;
-    const int y = (int)(c % (unsigned)ny);
-    c /= (unsigned)ny;
-    const int ie = (int)(c % (unsigned)ne);
-    const int z = (int)(c / (unsigned)ne);
-    const int e = -(sf_space_exp(x, y, z) + sf_energy_exp(e_off + ie) + sf_unknown_exp(u));
+                                                      unsigned short* __restrict__ lo, int nx,
+                                                      int ny, int ne, int nm_log2, int nu,
+                                                      int chunks, int e_off) {
+  const int plane = (int)blockIdx.x / chunks;  // z * ne + group
+  const int part = (int)blockIdx.x - plane * chunks;
+  const int ie = plane % ne, z = plane / ne;
+  const I cellv = (I)nu << nm_log2;            // values of one cell
+  const I nvals = (I)ny * (I)nx * cellv;       // values of the plane
+  const int ez = sf_energy_exp(e_off + ie);
+  const size_t base = (size_t)plane * nvals;
+  for (I off = ((I)part * blockDim.x + threadIdx.x) * 4; off < nvals;
+       off += (I)chunks * blockDim.x * 4) {
+    const I cell = off / cellv;                // y * nx + x
+    const int u = (int)((off >> nm_log2) - cell * (I)nu);
+    const int y = (int)(cell / (I)nx);
+    const int x = (int)(cell - (I)y * (I)nx);
+    const size_t i = base + off;
+    const int e = -(sf_space_exp(x, y, z) + ez + sf_unknown_exp(u));
     const double2 a = *reinterpret_cast<const double2*>(vi + i);
     const double2 b = *reinterpret_cast<const double2*>(vi + i + 2);
     const double v[4] = {__builtin_ldexp(a.x, e), __builtin_ldexp(a.y, e),
@@ -102,11 +110,21 @@ size_t operand_copy_bytes(int precision, size_t n) {
 void launch_operand_copy(int precision, const MswDims& d, const double* vi, float* hi,
                          unsigned short* lo, int e_off, hipStream_t st) {
   if (precision != MSW_PREC_MIXED_FP32 && precision != MSW_PREC_MIXED_BF16) return;
-  const size_t n = (size_t)d.ncell_x * d.ncell_y * d.ncell_z * d.ne * d.nm * d.nu;
-  const unsigned blocks = (unsigned)std::min<size_t>((n / 4 + 255) / 256, 8192);
-  hipLaunchKernelGGL(k_operand_copy, dim3(blocks), dim3(256), 0, st, vi,
-                     precision == MSW_PREC_MIXED_FP32 ? hi : nullptr, lo, n, d.ncell_x,
-                     d.ncell_y, d.ne, d.nm, d.nu, e_off);
+  const int nm_log2 = d.nm == 16 ? 4 : d.nm == 8 ? 3 : 2;  // nm is 4, 8 or 16
+  const long long nruns = (long long)d.ncell_x * d.ncell_y * d.nu * d.nm / 4;  // per plane
+  const long long planes = (long long)d.ncell_z * d.ne;
+  // about 4 runs per thread, and no more blocks than 2^31
+  long long chunks = std::max(1LL, (nruns + 1023) / 1024);
+  chunks = std::min(chunks, std::max(1LL, (long long)INT32_MAX / planes));
+  float* h = precision == MSW_PREC_MIXED_FP32 ? hi : nullptr;
+  if (nruns * 4 + (long long)chunks * 256 * 4 < (1LL << 32))
+    hipLaunchKernelGGL(k_operand_copy<unsigned>, dim3((unsigned)(planes * chunks)), dim3(256), 0,
+                       st, vi, h, lo, d.ncell_x, d.ncell_y, d.ne, nm_log2, d.nu, (int)chunks,
+                       e_off);
+  else
+    hipLaunchKernelGGL(k_operand_copy<unsigned long long>, dim3((unsigned)(planes * chunks)),
+                       dim3(256), 0, st, vi, h, lo, d.ncell_x, d.ncell_y, d.ne, nm_log2, d.nu,
+                       (int)chunks, e_off);
 }
 
 
