@@ -384,7 +384,7 @@ void batch_release(MswBatch* b) {
 int nac_for(const MswBatch* b, const MswDims& d) {
   auto it = b->class_nac.find(std::make_pair(d.nm, d.na));
   return it != b->class_nac.end() && it->second ? it->second
-                                                : default_mfma_nac(b->opt.precision, d.na);
+                                                : default_mfma_nac(b->opt.precision, d, false);
 }
 
 // Start ticks that even out the combined launches: problems with the most
@@ -614,7 +614,7 @@ int msw_batch_create(int nprob, const MswDims* dims, const MswBatchOptions* opt,
     return set_error(MSW_ERR_INVALID, "stream_target must be 0..4096");
   for (int i = 0; i < nprob; ++i) {
     Geom g;
-    const int rc = make_geom(dims[i], default_mfma_nac(o.precision, dims[i].na), &g);
+    const int rc = make_geom(dims[i], default_mfma_nac(o.precision, dims[i], false), &g);
     if (rc) return rc;
   }
   MswBatch* b = new MswBatch();
@@ -1030,7 +1030,8 @@ void server_loop(MswServer* s) {
         const auto cn = s->class_nac.find(std::make_pair(q.d.nm, q.d.na));
         int rc = make_geom(q.d,
                            cn != s->class_nac.end() ? cn->second
-                                                    : default_mfma_nac(s->opt.precision, q.d.na),
+                                                    : default_mfma_nac(s->opt.precision, q.d,
+                                                                       false),
                            &a.g, s->row_groups, s->opt.precision);
         if (!rc) {
           const int nz = a.g.d.ncell_z;

@@ -80,8 +80,11 @@ int set_error(int code, const std::string& msg);
 // Most energy groups one tile pipeline streams (the kernel keeps the groups'
 // scale exponents in one 64-bit word).
 constexpr int kMaxStreamGroups = 32;
-// Default angles per wave of the MFMA kernels for a precision and na.
-int default_mfma_nac(int precision, int na);
+// Default angles per wave of the MFMA kernels for a precision and problem;
+// alone: the problem's launches run by themselves (a plan), not combined with
+// other problems' (batch and server ticks, which keep one choice per class so
+// that a request's bits do not depend on what else is in flight).
+int default_mfma_nac(int precision, const MswDims& d, bool alone);
 // Energy groups per workgroup of the MFMA kernels: about 128 stream steps
 // per workgroup, then fewer while a launch of `waves1` waves at one group per
 // workgroup would drop below 4 waves per SIMD.

@@ -74,9 +74,25 @@ int launch_mfma_epw(int preferred, int ne, long long waves1, int precision) {
 // per wave is fastest in FP64 (16 loses more to the cross-wave vo reduction
 // than it gains in occupancy) and 64 in the mixed forms; never more than na
 // rounded up to a 16-angle block, and at most 4 waves per tile.
-int default_mfma_nac(int precision, int na) {
+int default_mfma_nac(int precision, const MswDims& d, bool alone) {
+  const int na = d.na;
   const int cap = na <= 16 ? 16 : na <= 32 ? 32 : 64;
   int nac = precision == MSW_PREC_FP64 ? 32 : 64;
+  // FP64 with at most 8192 (tile, group, unknown) streams, its launches run
+  // alone: 16 angles per wave, two waves per SIMD.  The extra operand loads
+  // and the reduction over more angle waves cost less than the second wave
+  // gains until the launches hold enough streams to hide latency at one wave
+  // (one MI355X, ms per sweep at 32 / 16 angles: 64^3 ne64 16384 streams
+  // 91.9 / 94.2; ne32 51.0 / 49.2; ne16 27.8 / 26.0; 48^3 ne32 23.4 / 22.0;
+  // 32^3 ne16 5.0 / 4.4; 16^3 ne8 1.44 / 1.03; the 256 uneven problems of
+  // the batch workload one after another 410 / 471 problems/s).  In launches
+  // combined with other problems' (batch ticks) the waves are there already:
+  // the 256 problems batched in FP64 run 743 / 598 problems/s.
+  if (precision == MSW_PREC_FP64 && alone) {
+    const long long tiles = (long long)((d.ncell_x + mfma_tile_x() - 1) / mfma_tile_x()) *
+                            ((d.ncell_y + mfma_tile_rows() - 1) / mfma_tile_rows());
+    if (tiles * d.ne * d.nu <= 8192) nac = 16;
+  }
   nac = nac < cap ? nac : cap;
   while ((na + nac - 1) / nac > 4 && nac < 64) nac *= 2;
   return nac;

@@ -599,7 +599,7 @@ int msw_plan_create_ex(const MswDims* d, const MswPlanOptions* opt, MswPlan** ou
   if (p->engine == MSW_ENGINE_MFMA) {
     // 16-angle MFMA blocks: 16, 32 or 64 angles per wave, at most 4 waves
     // (angle chunks) per cell tile
-    p->nac = opt->angles_per_wave ? opt->angles_per_wave : default_mfma_nac(precision, d->na);
+    p->nac = opt->angles_per_wave ? opt->angles_per_wave : default_mfma_nac(precision, *d, true);
     if (p->nac != 16 && p->nac != 32 && p->nac != 64) {
       delete p;
       return set_err(MSW_ERR_INVALID,

@@ -8,7 +8,10 @@ global group index (src/3_sweeper/quantities_testing_kernels.h:52-70).  So a pro
 groups splits into contiguous group ranges, one per rank.  A rank keeps the
 [z][its groups] slabs of vi and vo as a state of its own ne_r groups, in the
 usual layout, and sweeps it with a plan whose ``energy_offset`` is its first
-group; its vo is, bit for bit, that slab of the whole problem's vo.  The only
+group; at the same angles per wave its vo is, bit for bit, that slab of the
+whole problem's vo (a plan's FP64 default depends on the problem's size, so
+a share can default to 16 angles per wave where the whole takes 32; pass
+``angles_per_wave`` to pin it).  The only
 exchange per sweep is the norms: each rank's FP64 normsq and normsqdiff
 partials, summed by one all-reduce of two values (the reference's
 get_state_norms over all groups, src/4_driver/runner.c:173-174 and

@@ -42,9 +42,13 @@ def _inputs(torch, dev, shapes, seed=7, perturb=True):
     return out
 
 
-def _single(torch, d, vi, a, m, prec):
+def _single(torch, d, vi, a, m, prec, batch=None):
+    """vo of a single plan sweep; with the angles per wave the batch gives the
+    problem's (nm, na) class when a batch is named (a plan's own default can
+    differ: FP64 plans of small problems take 16 angles per wave, batches 32)."""
     vo = torch.full_like(vi, float("nan"))
-    plan = msw.Plan(d, prec, engine=msw.ENGINE_MFMA)
+    nac = batch.describe()["class_nac"][f"nm{d.nm}_na{d.na}"] if batch is not None else 0
+    plan = msw.Plan(d, prec, engine=msw.ENGINE_MFMA, angles_per_wave=nac)
     plan.sweep(vi, vo, a, m)
     torch.cuda.synchronize()
     plan.close()
@@ -64,7 +68,7 @@ def test_batch_matches_single_sweeps(prec, graph, cuda):
     b.run()
     torch.cuda.synchronize()
     for d, vi0, vi, vo, a, m in ins:
-        np.testing.assert_array_equal(vo.cpu().numpy(), _single(torch, d, vi, a, m, prec))
+        np.testing.assert_array_equal(vo.cpu().numpy(), _single(torch, d, vi, a, m, prec, b))
     if prec == msw.PREC_FP64:
         for d, vi0, vi, vo, a, m in ins[:3]:
             od = oracle.dims(*d.tuple())
@@ -265,7 +269,8 @@ def test_batch_more_problems_than_one_launch_holds(cuda):
         n, nd = msw.state_norms(d, vi, vo)
         assert nd == 0.0 and n == float(closed_form_normsq(s[0], s[1], s[2], s[3], s[4], s[6]))
     d, vi0, vi, vo, a, m = ins[7]
-    np.testing.assert_array_equal(vo.cpu().numpy(), _single(torch, d, vi, a, m, msw.PREC_FP64))
+    np.testing.assert_array_equal(vo.cpu().numpy(),
+                                  _single(torch, d, vi, a, m, msw.PREC_FP64, b))
     b.close()
 
 
